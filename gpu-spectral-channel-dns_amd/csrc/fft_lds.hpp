@@ -1,0 +1,188 @@
+// fft_lds.hpp -- batched mixed-radix (8/4/2/3/5) Stockham FFT on sequences held in
+// LDS, FP64 (double2) and FP32 (float2).  One workgroup transforms B sequences of
+// length n stored at buf + b*ld; every stage reads R inputs per butterfly into
+// registers, synchronises, and writes the R outputs back in place (self-sorting
+// Stockham order, Govindaraju et al., SC'08).  Unnormalised: DIR = -1 computes
+// sum_j x_j e^{-2 pi i jk/n}, DIR = +1 the inverse sign -- the DFTs the reference
+// takes from cuFFT (source/transform.cu:49-71).
+#pragma once
+#include <hip/hip_runtime.h>
+
+// A thread holds at most RPCF_REGC complex values of one stage in registers: a radix-R
+// stage lets each thread process RPCF_REGC/R butterflies, so one block transforms at
+// most RPCF_REGC*nthr complex elements (B*n <= fft_capacity(plan, nthr), checked on the
+// host before every launch).
+#ifndef RPCF_REGC
+#define RPCF_REGC 16
+#endif
+
+__host__ __device__ inline int fft_capacity(const int* rad, int nst, int nthr) {
+    int cap = 1 << 30;
+    for (int s = 0; s < nst; s++) {
+        const int R = rad[s];
+        const int c = (RPCF_REGC / R) * R * nthr;
+        cap = c < cap ? c : cap;
+    }
+    return cap;
+}
+
+struct FftPlan {
+    int n;
+    int nst;
+    int rad[16];
+    const void* tw;  // n twiddles exp(-2 pi i k/n), double2 or float2
+};
+
+template <typename T> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+
+template <typename C> __device__ __forceinline__ C cmake(decltype(C().x) a, decltype(C().x) b) { C c; c.x = a; c.y = b; return c; }
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmake<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmake<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) { return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C().x) s) { return cmake<C>(a.x * s, a.y * s); }
+// multiply by i (DIR=+1) or -i (DIR=-1)
+template <int DIR, typename C> __device__ __forceinline__ C cjrot(C a) {
+    return DIR > 0 ? cmake<C>(-a.y, a.x) : cmake<C>(a.y, -a.x);
+}
+
+template <int R, int DIR, typename C> struct Dft;
+
+template <int DIR, typename C> struct Dft<2, DIR, C> {
+    static __device__ __forceinline__ void run(C* v) {
+        C a = v[0];
+        v[0] = cadd(a, v[1]);
+        v[1] = csub(a, v[1]);
+    }
+};
+
+template <int DIR, typename C> struct Dft<4, DIR, C> {
+    static __device__ __forceinline__ void run(C* v) {
+        C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        C s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        C jd = cjrot<DIR>(d13);
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, jd);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, jd);
+    }
+};
+
+template <int DIR, typename C> struct Dft<3, DIR, C> {
+    static __device__ __forceinline__ void run(C* v) {
+        using R = decltype(C().x);
+        const R h = (R)0.86602540378443864676;  // sqrt(3)/2
+        C t1 = cadd(v[1], v[2]);
+        C t2 = csub(v[0], cscale(t1, (R)0.5));
+        C d = cscale(csub(v[1], v[2]), h);
+        C jd = cjrot<DIR>(d);
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(t2, jd);
+        v[2] = csub(t2, jd);
+    }
+};
+
+template <int DIR, typename C> struct Dft<5, DIR, C> {
+    static __device__ __forceinline__ void run(C* v) {
+        using R = decltype(C().x);
+        const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+        const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+        C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        C t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        C a1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+        C a2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+        C b1 = cadd(cscale(t3, s1), cscale(t4, s2));
+        C b2 = csub(cscale(t3, s2), cscale(t4, s1));
+        C jb1 = cjrot<DIR>(b1), jb2 = cjrot<DIR>(b2);
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(a1, jb1);
+        v[4] = csub(a1, jb1);
+        v[2] = cadd(a2, jb2);
+        v[3] = csub(a2, jb2);
+    }
+};
+
+template <int DIR, typename C> struct Dft<8, DIR, C> {
+    static __device__ __forceinline__ void run(C* v) {
+        using R = decltype(C().x);
+        const R h = (R)0.70710678118654752440;  // sqrt(2)/2
+        C e[4] = {v[0], v[2], v[4], v[6]};
+        C o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, DIR, C>::run(e);
+        Dft<4, DIR, C>::run(o);
+        // o_k *= exp(DIR * i pi k / 4)
+        C o1 = cmul(o[1], cmake<C>(h, (R)DIR * h));
+        C o2 = cjrot<DIR>(o[2]);
+        C o3 = cmul(o[3], cmake<C>(-h, (R)DIR * h));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+template <int R, int DIR, typename C>
+__device__ __forceinline__ void fft_stage(C* buf, int ld, int B, int n, int Ns, const C* __restrict__ tw, int tid,
+                                          int nthr) {
+    const int nR = n / R;
+    const int tot = B * nR;
+    const int tstep = n / (Ns * R);
+    constexpr int MAXPT = RPCF_REGC / R;
+    C v[MAXPT][R];
+#pragma unroll
+    for (int it = 0; it < MAXPT; it++) {
+        int t = tid + it * nthr;
+        if (t < tot) {
+            int b = t / nR, j = t - b * nR, k = j % Ns;
+            const C* base = buf + b * ld;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                C x = base[j + r * nR];
+                if (r > 0) {
+                    C w = tw[k * r * tstep];
+                    if (DIR > 0) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[it][r] = x;
+            }
+            Dft<R, DIR, C>::run(v[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MAXPT; it++) {
+        int t = tid + it * nthr;
+        if (t < tot) {
+            int b = t / nR, j = t - b * nR, k = j % Ns;
+            C* base = buf + b * ld;
+            int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) base[d + r * Ns] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Transform B sequences buf[b*ld .. b*ld+n) in place.  Callers synchronise before
+// (data written) and may read the result right after (the last stage ends with a barrier).
+template <int DIR, typename C>
+__device__ void fft_run(C* buf, int ld, int B, const FftPlan& pl, int tid, int nthr) {
+    const C* tw = (const C*)pl.tw;
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; s++) {
+        const int R = pl.rad[s];
+        switch (R) {
+            case 8: fft_stage<8, DIR>(buf, ld, B, pl.n, Ns, tw, tid, nthr); break;
+            case 4: fft_stage<4, DIR>(buf, ld, B, pl.n, Ns, tw, tid, nthr); break;
+            case 2: fft_stage<2, DIR>(buf, ld, B, pl.n, Ns, tw, tid, nthr); break;
+            case 3: fft_stage<3, DIR>(buf, ld, B, pl.n, Ns, tw, tid, nthr); break;
+            default: fft_stage<5, DIR>(buf, ld, B, pl.n, Ns, tw, tid, nthr); break;
+        }
+        Ns *= R;
+    }
+}

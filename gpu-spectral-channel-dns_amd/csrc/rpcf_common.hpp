@@ -1,0 +1,25 @@
+// rpcf_common.hpp -- sizes and small helpers shared by the HIP kernels and the host code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+// Grid sizes in the reference's naming (include/data.h:189-334, source/data.cpp:20-32):
+// mx, my padded x / span lengths; nx = mx/3*2, ny = my/3*2 kept modes; nz = mz/4+1
+// Chebyshev coefficients, N = nz-1; pz = mz/2+1 = 2N+1 padded wall-normal points;
+// hnx = nx/2+1 kx pencils (the last one is the x-Nyquist, zeroed on the way to physical
+// space); nxh = nx/2 kx columns that reach physical space; P = hnx*ny pencils.
+struct Dims {
+    int mx, my, nx, ny, nz, N, pz, hnx, nxh, P;
+    int nslot;  // hnx*(ny/2+1) matrix slots: pencils ky and ny-ky share one matrix
+};
+
+// physical span row of pencil ky (source/transpose.cu:238-240)
+__host__ __device__ __forceinline__ int span_of(const Dims& d, int ky) {
+    return ky > d.ny / 2 ? ky + (d.my - d.ny) : ky;
+}
+// pencil ky of a physical span row, or -1 if the row is zeroed by the dealiasing mask
+// (source/transform.cu:366-368: rows ny/2 .. my-ny/2 are zero on the way back)
+__host__ __device__ __forceinline__ int ky_of_span(const Dims& d, int s) {
+    if (s < d.ny / 2) return s;
+    if (s > d.my - d.ny / 2) return s - (d.my - d.ny);
+    return -1;
+}
