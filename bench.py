@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of one time step of the Fourier-Chebyshev channel DNS on one MI355X.
+
+Workload (the headline configuration): pressure-driven channel 128x129x128 (mx = my =
+192, mz = 512: padded 192 x 257 x 192), Lx = 4 pi, Lz = 4 pi/3, Re = 3300, constant flow
+rate, dt = 0.002, U = 1 - y^2, seeded wall-satisfying perturbation (amplitude 0.1,
+|k| <= 16, degree <= 24, (1+|k|)^(-5/6), seed 0).  A step is nonlinear term +
+Crank-Nicolson/AB2 solves + velocity recovery, all resident in HBM.
+
+Prints one JSON line: ns per grid point per step (Nx*Ny*Nz = 2,113,536 points), the
+roofline of the dominant kernel from HIP events recorded on the library's stream around
+every launch of the timed steps, and the CPU baseline (the FP64 oracle, OpenMP, on this
+host) on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-spectral-channel-dns_amd"))
+
+import numpy as np  # noqa: E402
+
+import rpcf  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+C1 = dict(mx=192, my=192, mz=512, n_pi_x=4.0, n_pi_y=4.0 / 3.0, Re=3300.0, Ro=0.0, dt=0.002,
+          u_profile=(1.0, 0.0, -1.0), flow_control=1)
+C1_IC = (0.1, 16, 24, 5.0 / 6.0)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def kernel_bytes(s, prm):
+    """algorithmic HBM bytes per launch of each kernel: the arrays it must read and
+    write once (complex FP64 = 16 B)."""
+    P, nz, hnx, ny, pz = s["P"], s["nz"], s["hnx"], s["ny"], s["pz"]
+    my, nxh = prm.my, s["nx"] // 2
+    S = P * nz * 16                    # one spectral field
+    Sp = hnx * ny * pz * 16            # one field on the padded wall-normal points
+    Wb = pz * my * nxh * 16            # one field after the span transform (kx < nx/2)
+    Zb = pz * my * hnx * 16            # one Lamb component after the x transform
+    Mu = hnx * (ny // 2 + 1) * nz * nz * 16  # one set of inverses, pencils +-ky shared
+    return {
+        "k_cheb_s2p": 6 * S + 6 * Sp,
+        "k_span_inv": 6 * Sp + 6 * Wb,
+        "k_lamb_phys": 6 * Wb + 3 * Zb,
+        "k_span_fwd": 3 * Zb + 3 * Sp,
+        "k_p2s_nl": 3 * Sp + 2 * S,
+        "k_linear": 2 * Mu + 8 * S,
+        "k_velocity": 8 * S,
+    }
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+    committed in profiles/traffic_<kernel>_c1.txt (line 'traffic_bytes_per_launch: N')."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_c1.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(prm, v0, e0, steps, ref_state):
+    """the FP64 oracle (oracle/rpcf_oracle.c) on this host's cores: create (matrix
+    build, not timed), bootstrap, then `steps` timed steps"""
+    dll = rpcf.load(rpcf.ORACLE_LIB)
+    dll.rpcf_oracle_threads.restype = ctypes.c_int
+    threads = int(dll.rpcf_oracle_threads())
+    t0 = time.perf_counter()
+    with rpcf.Simulation(rpcf.ORACLE_LIB, prm) as o:
+        t_init = time.perf_counter() - t0
+        o.set_state(v0, e0)
+        o.bootstrap()
+        a = o.get_state()
+        parity = max(float(np.abs(ref_state[i] - a[i]).max() / np.abs(a[i]).max()) for i in (0, 1))
+        t0 = time.perf_counter()
+        o.step(steps)
+        t_steps = time.perf_counter() - t0
+    return threads, t_init, t_steps / steps, parity
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs one simulation on one GPU; multi-GPU runs are not "
+              "part of it (the pencil/plane slab decomposition is design only). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    prm = rpcf.make_params(**C1)
+    t0 = time.perf_counter()
+    sim = rpcf.Simulation(rpcf.HIP_LIB, prm)
+    t_create = time.perf_counter() - t0
+    s = sim.s
+    npts = s["nx"] * s["nz"] * s["ny"]
+    v0, e0 = rpcf.seed_state(sim.dll, prm, 0, *C1_IC)
+    sim.set_state(v0, e0)
+    sim.bootstrap()
+    state1 = sim.get_state()
+    sim.step(args.warmup)
+    log(f"create {t_create:.2f}s; grid {s}")
+
+    dll = sim.dll
+    dll.rpcf_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    dll.rpcf_timing_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_long)]
+    dll.rpcf_kernel_name.restype = ctypes.c_char_p
+    dll.rpcf_kernel_name.argtypes = [ctypes.c_int]
+    assert dll.rpcf_timing(sim.h, 1) == 0
+
+    device_sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed / args.steps * 1e3
+
+    nbytes = kernel_bytes(s, prm)
+    kernels = {}
+    for kid in range(dll.rpcf_num_kernels()):
+        name = dll.rpcf_kernel_name(kid).decode()
+        tot, cnt = ctypes.c_double(), ctypes.c_long()
+        dll.rpcf_timing_get(sim.h, kid, ctypes.byref(tot), ctypes.byref(cnt))
+        if cnt.value == 0:
+            continue
+        avg = tot.value / cnt.value
+        gbs = nbytes[name] / (avg * 1e-3) / 1e9
+        kernels[name] = {"avg_ms": round(avg, 5), "launches": cnt.value, "bytes": nbytes[name],
+                         "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "share": round(tot.value / (elapsed * 1e3), 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    dk = kernels[dom]
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": dk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": dk["frac"], "traffic": read_traffic(dom)}
+    sim.close()
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        threads, t_init, t_cpu, parity = cpu_baseline(prm, v0, e0, args.cpu_steps, state1)
+        cpu = {"value": round(t_cpu * 1e9 / npts, 2), "unit": "ns/gridpoint/step", "cores": threads,
+               "kind": "port", "sample": f"FP64 oracle (OpenMP, {threads} threads) on the same 128x129x128 "
+                                         f"channel: matrix build {t_init:.1f}s untimed, bootstrap, then "
+                                         f"{args.cpu_steps} timed steps ({t_cpu * 1e3:.0f} ms/step)"}
+        log(f"cpu baseline {t_cpu * 1e3:.1f} ms/step on {threads} threads; GPU-vs-oracle parity after 1 step {parity:.2e}")
+
+    out = {
+        "metric": "ns/gridpoint/step at 128x129x128 (FP64)",
+        "value": round(ms_step * 1e6 / npts, 4),
+        "unit": "ns/gridpoint/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded wall-satisfying perturbation, seed 0)",
+        "config": {"workload": "channel 128x129x128 FP64, constant flow rate", "grid": "128x129x128",
+                   "padded": "192x257x192", "Re": 3300, "dt": 0.002, "Lx": "4pi", "Lz": "4pi/3",
+                   "parallelism": "single GPU"},
+        "steps_per_s": round(1e3 / ms_step, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity_1step_vs_oracle": parity,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

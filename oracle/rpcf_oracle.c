@@ -1198,3 +1198,12 @@ int rpcf_timing_get(rpcf_ctx* c, int kid, double* total_ms, long* count) {
 }
 const char* rpcf_kernel_name(int kid) { (void)kid; return NULL; }
 int rpcf_num_kernels(void) { return 0; }
+
+/* number of OpenMP threads the oracle uses (reported with the CPU baseline) */
+int rpcf_oracle_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

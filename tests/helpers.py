@@ -1,0 +1,74 @@
+"""Shared test helpers: workload parameter sets, physical grids, norms.
+
+Workloads (grids in the reference's mx, my, mz units; nz = mz/4+1):
+  REF128  the reference's input/parameter.txt grid 128x128x128, box 4pi x 2pi, used
+          by its closed-form tests (tests/test_transform.cpp etc.)
+  C0      32x33x32 plane Couette, U = y, Lx = 2pi, Lz = pi, Re = 1000, dt = 0.005
+  C1      128x129x128 channel, U = 1 - y^2, Lx = 4pi, Lz = 4pi/3, Re = 3300, dt = 0.002,
+          constant flow rate
+  C2      256x97x256 rotating Couette, Ro = 0.1, Lx = 4pi, Lz = 2pi, Re = 1300, dt = 0.002
+"""
+import numpy as np
+
+import rpcf
+
+
+def params(name, **over):
+    base = {
+        "REF128": dict(mx=128, my=128, mz=128, n_pi_x=4.0, n_pi_y=2.0, Re=2600.0, Ro=0.0, dt=0.005),
+        "REFCOUETTE": dict(mx=48, my=48, mz=128, n_pi_x=4.0, n_pi_y=2.0, Re=2600.0, Ro=0.015, dt=0.005,
+                           u_profile=(0.5, 0.5, 0.0)),
+        "C0": dict(mx=48, my=48, mz=128, n_pi_x=2.0, n_pi_y=1.0, Re=1000.0, Ro=0.0, dt=0.005,
+                   u_profile=(0.0, 1.0, 0.0)),
+        "C1": dict(mx=192, my=192, mz=512, n_pi_x=4.0, n_pi_y=4.0 / 3.0, Re=3300.0, Ro=0.0, dt=0.002,
+                   u_profile=(1.0, 0.0, -1.0), flow_control=1),
+        "C2": dict(mx=384, my=384, mz=384, n_pi_x=4.0, n_pi_y=2.0, Re=1300.0, Ro=0.1, dt=0.002,
+                   u_profile=(0.0, 1.0, 0.0)),
+        "TINY": dict(mx=24, my=24, mz=64, n_pi_x=2.0, n_pi_y=1.0, Re=500.0, Ro=0.2, dt=0.01,
+                     u_profile=(0.0, 1.0, 0.0)),
+    }[name]
+    base.update(over)
+    return rpcf.make_params(**base)
+
+
+# seeded initial-field recipes (amplitude, kmax, degmax, kexp)
+IC = {
+    "C0": (1e-2, 4, 8, 0.0),
+    "C1": (0.1, 16, 24, 5.0 / 6.0),
+    "C2": (0.05, 16, 24, 5.0 / 6.0),
+    "TINY": (5e-2, 4, 8, 0.0),
+    "REFCOUETTE": (1e-2, 4, 8, 0.0),
+}
+
+
+def rel(a, b):
+    """normwise relative difference max|a-b| / max|b| (the parity metric)"""
+    den = float(np.abs(b).max())
+    return float(np.abs(a - b).max()) / (den if den > 0 else 1.0)
+
+
+def grid(prm, planes):
+    """physical coordinates (z wall-normal, y span, x) as in the reference tests"""
+    lx, ly = np.pi * prm.n_pi_x, np.pi * prm.n_pi_y
+    z = np.cos(np.arange(planes) / (planes - 1) * np.pi)
+    y = ly * np.arange(prm.my) / prm.my
+    x = lx * np.arange(prm.mx) / prm.mx
+    return np.meshgrid(z, y, x, indexing="ij")
+
+
+def pencils_to_phys(prm, s, vals):
+    """values on wall-normal points in pencil layout (P, planes) -> physical field,
+    with the reference's dealiasing mask and an unnormalised C2R (numpy irfft2 x my*mx)"""
+    planes = vals.shape[1]
+    mx, my, ny, hnx, nx = prm.mx, prm.my, s["ny"], s["hnx"], s["nx"]
+    H = np.zeros((planes, my, mx // 2 + 1), complex)
+    for ky in range(ny):
+        row = ky if ky <= ny // 2 else ky + my - ny
+        H[:, row, :hnx] = vals[ky * hnx:(ky + 1) * hnx, :].T
+    H[:, ny // 2:my - ny // 2 + 1, :] = 0
+    H[:, :, nx // 2:] = 0
+    return np.fft.irfft2(H, s=(my, mx)) * (my * mx)
+
+
+def pencil_index(s, kx, ky):
+    return ky * s["hnx"] + kx

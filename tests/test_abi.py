@@ -1,0 +1,78 @@
+"""The C ABI boundary: both libraries load and export every entry point declared in
+include/rpcf.h; argument checking and error reporting work without a GPU (calls that
+need no device only).  CPU only."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import rpcf
+from helpers import IC, params
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "rpcf.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rpcf_\w+)\s*\(", src)))
+
+
+def test_header_declares_the_step_api():
+    names = declared()
+    for n in ["rpcf_create", "rpcf_destroy", "rpcf_set_state", "rpcf_get_state", "rpcf_nonlinear", "rpcf_linear",
+              "rpcf_velocity", "rpcf_step", "rpcf_bootstrap", "rpcf_last_error", "rpcf_seed_state"]:
+        assert n in names
+
+
+@pytest.mark.parametrize("which", ["oracle", "hip"])
+def test_library_exports_every_declared_symbol(which):
+    path = rpcf.ORACLE_LIB if which == "oracle" else rpcf.HIP_LIB
+    dll = ctypes.CDLL(path)
+    missing = [n for n in declared() if not hasattr(dll, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("which", ["oracle", "hip"])
+def test_sizes_and_rejections(which):
+    dll = rpcf.load(rpcf.ORACLE_LIB if which == "oracle" else rpcf.HIP_LIB)
+    s = rpcf.sizes(dll, params("C1"))
+    assert (s["nx"], s["ny"], s["nz"], s["pz"], s["hnx"]) == (128, 128, 129, 257, 65)
+    s = rpcf.sizes(dll, params("REF128"))
+    assert (s["nx"], s["ny"], s["nz"], s["pz"]) == (84, 84, 33, 65)
+    for bad in (dict(mz=130), dict(mx=47), dict(Re=0.0), dict(dt=-1.0)):
+        with pytest.raises(rpcf.RpcfError):
+            rpcf.sizes(dll, params("C0", **bad))
+    out = ctypes.c_void_p()
+    assert dll.rpcf_create(ctypes.byref(params("C0", mz=130)), ctypes.byref(out)) == -1
+    assert dll.rpcf_create(None, ctypes.byref(out)) == -1
+
+
+def test_seed_identical_in_both_libraries():
+    """the HIP library's host-side generator and the oracle's restatement agree bit for bit"""
+    for name in ("C0", "C1", "TINY"):
+        prm = params(name)
+        a = rpcf.seed_state(rpcf.load(rpcf.ORACLE_LIB), prm, 5, *IC[name])
+        b = rpcf.seed_state(rpcf.load(rpcf.HIP_LIB), prm, 5, *IC[name])
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_seed_rejects_too_high_degree():
+    prm = params("TINY")
+    for path in (rpcf.ORACLE_LIB, rpcf.HIP_LIB):
+        with pytest.raises(rpcf.RpcfError):
+            rpcf.seed_state(rpcf.load(path), prm, 0, 1.0, 4, 13, 0.0)  # nz = 17: degree + 4 must be < nz
+
+
+def test_oracle_errors_are_reported_not_fatal():
+    prm = params("TINY")
+    with rpcf.Simulation(rpcf.ORACLE_LIB, prm) as sim:
+        with pytest.raises(rpcf.RpcfError, match="no state"):
+            sim.step(1)
+        with pytest.raises(rpcf.RpcfError, match="bad field"):
+            sim.get_field(7)
+        z = np.zeros((sim.s["P"], sim.s["nz"]), np.complex128)
+        sim.set_state(z, z)
+        sim.step(1)

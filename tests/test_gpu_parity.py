@@ -1,0 +1,150 @@
+"""Parity of the HIP library (librpcf_hip.so, MI355X) with the CPU oracle on the same
+seeded inputs, through the C ABI.  Metric: per field max|a-b|/max|b| over all pencils,
+the carried Nyquist pencils included.  Bars: 1e-12 after one step, 1e-9 after 100
+steps (FP64); matrices and velocity recovery bit for bit."""
+import numpy as np
+import pytest
+
+import rpcf
+from helpers import IC, grid, params, pencil_index, pencils_to_phys, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(prm):
+    return rpcf.Simulation(rpcf.HIP_LIB, prm), rpcf.Simulation(rpcf.ORACLE_LIB, prm)
+
+
+@pytest.mark.parametrize("name", ["C0", "TINY"])
+def test_inverse_matrices_bit_identical(name):
+    g, o = pair(params(name))
+    s = g.s
+    for eq in (0, 1):
+        for p in range(s["P"]):
+            assert np.array_equal(g.get_inverse(eq, p), o.get_inverse(eq, p)), (eq, p)
+    g.close(), o.close()
+
+
+def test_inverse_matrices_c1_sample():
+    g, o = pair(params("C1"))
+    s = g.s
+    for eq in (0, 1):
+        for kx, ky in [(0, 0), (1, 0), (0, 1), (64, 64), (17, 100), (33, 5)]:
+            p = pencil_index(s, kx, ky)
+            assert np.array_equal(g.get_inverse(eq, p), o.get_inverse(eq, p)), (eq, kx, ky)
+    g.close(), o.close()
+
+
+ODD = dict(mx=60, my=30, mz=80, n_pi_x=3.0, n_pi_y=1.5, Re=400.0, Ro=0.1, dt=0.01, u_profile=(0.0, 1.0, 0.0))
+
+
+@pytest.mark.parametrize("name", ["C0", "REF128", "TINY", "ODD"])
+def test_transforms_match_oracle(name):
+    prm = rpcf.make_params(**ODD) if name == "ODD" else params(name)
+    g, o = pair(prm)
+    s = g.s
+    rng = np.random.default_rng(1)
+    for padded in (True, False):
+        planes = s["pz"] if padded else s["nz"]
+        spec = (rng.standard_normal((s["P"], s["nz"])) + 1j * rng.standard_normal((s["P"], s["nz"])))
+        assert rel(g.spec_to_phys(spec, padded), o.spec_to_phys(spec, padded)) < 1e-13
+        phys = rng.standard_normal((planes, prm.my, prm.mx))
+        assert rel(g.phys_to_spec(phys, padded), o.phys_to_spec(phys, padded)) < 1e-13
+    g.close(), o.close()
+
+
+def test_closed_forms_on_gpu():
+    """the reference's GPU closed-form tests (test_transform.cpp, test_nonlinear.cpp,
+    test_getUVW.cpp) run against the HIP library, tolerance 1e-8"""
+    prm = params("REF128")
+    with rpcf.Simulation(rpcf.HIP_LIB, prm) as sim:
+        s = sim.s
+        Z, Y, X = grid(prm, s["pz"])
+        Zn, Yn, Xn = grid(prm, s["nz"])
+        f = (1 - Z * Z) * np.sin(Y) * np.cos(X)
+        spec = sim.phys_to_spec(f, True)
+        assert np.abs(sim.spec_to_phys(spec, True) - f).max() < 1e-8
+        assert np.abs(sim.spec_to_phys(spec, False) - (1 - Zn * Zn) * np.sin(Yn) * np.cos(Xn)).max() < 1e-8
+        zero = np.zeros_like(Z)
+        fields = [f, zero, zero, zero, -2 * Z * np.sin(Y) * np.cos(X), -(1 - Z * Z) * np.cos(Y) * np.cos(X)]
+        for i, fl in enumerate(fields):
+            sim.set_field(i, sim.phys_to_spec(fl, True))
+        sim.nonlinear()
+        nv, ne = sim.get_nonlinear()
+        z, y, x = Zn, Yn, Xn
+        ex_v = -(1 - z * z) * (2 * z) * (np.cos(2 * x) - np.cos(2 * y) - 2 * np.cos(2 * y) * np.cos(2 * x)) + 2 * (
+            1 - z * z) * (-2 * z) * np.cos(2 * y) * np.cos(x) ** 2
+        ex_e = -(1 - z * z) ** 2 * np.cos(y) * np.sin(y) * 2 * np.cos(x) * np.sin(x)
+        assert np.abs(pencils_to_phys(prm, s, nv) - ex_v).max() < 1e-8
+        assert np.abs(pencils_to_phys(prm, s, ne) - ex_e).max() < 1e-8
+        flds = [np.sin(X), np.cos(Y), -Z * np.cos(X) + Z * np.sin(Y), -Z * np.cos(Y), Z * np.sin(X), zero]
+        sp = [sim.phys_to_spec(fl, True) for fl in flds]
+        sim.set_state(sp[2], sp[5])
+        for i, fl in enumerate(flds):
+            assert np.abs(sim.spec_to_phys(sim.get_field(i), True) - fl).max() < 1e-8, i
+
+
+@pytest.mark.parametrize("name", ["C0", "REFCOUETTE", "TINY"])
+def test_velocity_and_nonlinear_match(name):
+    prm = params(name)
+    g, o = pair(prm)
+    v0, e0 = rpcf.seed_state(o.dll, prm, 0, *IC[name])
+    g.set_state(v0, e0), o.set_state(v0, e0)
+    for f in range(6):
+        assert np.array_equal(g.get_field(f), o.get_field(f)), f
+    g.nonlinear(), o.nonlinear()
+    a, b = g.get_nonlinear(), o.get_nonlinear()
+    assert rel(a[0], b[0]) < 1e-13 and rel(a[1], b[1]) < 1e-13
+    g.close(), o.close()
+
+
+@pytest.mark.parametrize("name", ["C0", "REFCOUETTE", "TINY"])
+def test_one_and_hundred_steps(name):
+    prm = params(name)
+    g, o = pair(prm)
+    v0, e0 = rpcf.seed_state(o.dll, prm, 0, *IC[name])
+    for sim in (g, o):
+        sim.set_state(v0, e0)
+        sim.bootstrap()
+    a, b = g.get_state(), o.get_state()
+    print(name, "1 step", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
+    g.step(99), o.step(99)
+    a, b = g.get_state(), o.get_state()
+    print(name, "100 steps", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-9 and rel(a[1], b[1]) < 1e-9
+    g.close(), o.close()
+
+
+def test_channel_c1_one_step():
+    """the headline grid 128x129x128 channel with constant flow rate"""
+    prm = params("C1")
+    g, o = pair(prm)
+    v0, e0 = rpcf.seed_state(o.dll, prm, 0, *IC["C1"])
+    for sim in (g, o):
+        sim.set_state(v0, e0)
+        sim.bootstrap()
+    a, b = g.get_state(), o.get_state()
+    print("C1 1 step", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
+    g.step(2), o.step(2)
+    a, b = g.get_state(), o.get_state()
+    print("C1 3 steps", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-11 and rel(a[1], b[1]) < 1e-11
+    g.close(), o.close()
+
+
+def test_orr_sommerfeld_growth_rate_gpu():
+    from os_mode import growth_rate_run
+
+    sigma = growth_rate_run(rpcf.HIP_LIB)
+    print("TS growth rate", sigma)
+    assert abs(sigma / 0.00373967 - 1) < 1e-4
+
+
+def test_gpu_errors_are_reported():
+    with rpcf.Simulation(rpcf.HIP_LIB, params("TINY")) as sim:
+        with pytest.raises(rpcf.RpcfError, match="no state"):
+            sim.bootstrap()
+        with pytest.raises(rpcf.RpcfError, match="bad field"):
+            sim.set_field(9, np.zeros((sim.s["P"], sim.s["nz"]), np.complex128))
