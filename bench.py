@@ -54,14 +54,15 @@ def kernel_bytes(s, prm):
     Sp = hnx * ny * pz * 16            # one field on the padded wall-normal points
     Wb = pz * my * nxh * 16            # one field after the span transform (kx < nx/2)
     Zb = pz * my * hnx * 16            # one Lamb component after the x transform
-    Mu = hnx * (ny // 2 + 1) * nz * nz * 16  # one set of inverses, pencils +-ky shared
+    M1 = nz * nz * 16                  # one inverse matrix
+    nslot = hnx * (ny // 2 + 1)        # matrix slots: pencils +-ky share one
     return {
         "k_cheb_s2p": 6 * S + 6 * Sp,
         "k_span_inv": 6 * Sp + 6 * Wb,
         "k_lamb_phys": 6 * Wb + 3 * Zb,
         "k_span_fwd": 3 * Zb + 3 * Sp,
         "k_p2s_nl": 3 * Sp + 2 * S,
-        "k_linear": 2 * Mu + 8 * S,
+        "k_linear": 2 * nslot * M1 + 8 * S,
         "k_velocity": 8 * S,
     }
 

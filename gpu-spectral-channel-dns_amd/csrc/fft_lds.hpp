@@ -186,3 +186,90 @@ __device__ void fft_run(C* buf, int ld, int B, const FftPlan& pl, int tid, int n
         Ns *= R;
     }
 }
+
+// ---- compile-time plans ---------------------------------------------------------------
+// The same factorisation as the host (rpcf_host::factorize): as many 8s as possible, a
+// 4 (or 4,4 in place of an 8 and a 2), then 3s and 5s.  With the length, the radices and
+// every stage stride known at compile time, the index arithmetic of a stage reduces to
+// constant multiplies and shifts.
+constexpr int ct_pow2(int n) { int a = 0; while (n % 2 == 0) { n /= 2; a++; } return a; }
+constexpr int ct_odd(int n) { while (n % 2 == 0) n /= 2; return n; }
+constexpr int ct_nst(int n) {
+    int a = ct_pow2(n), q = a / 3, r = a % 3, k = 0;
+    if (r == 1 && q >= 1) k = (q - 1) + 2;
+    else k = q + (r != 0 ? 1 : 0);
+    int o = ct_odd(n);
+    while (o % 3 == 0) { o /= 3; k++; }
+    while (o % 5 == 0) { o /= 5; k++; }
+    return k;
+}
+constexpr int ct_rad(int n, int s) {
+    int a = ct_pow2(n), q = a / 3, r = a % 3;
+    int rad[16] = {0}, k = 0;
+    if (r == 1 && q >= 1) { for (int i = 0; i < q - 1; i++) rad[k++] = 8; rad[k++] = 4; rad[k++] = 4; }
+    else { for (int i = 0; i < q; i++) rad[k++] = 8; if (r == 1) rad[k++] = 2; if (r == 2) rad[k++] = 4; }
+    int o = ct_odd(n);
+    while (o % 3 == 0) { o /= 3; rad[k++] = 3; }
+    while (o % 5 == 0) { o /= 5; rad[k++] = 5; }
+    return rad[s];
+}
+
+template <int R, int DIR, int N, int NS, typename C>
+__device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __restrict__ tw, int tid, int nthr) {
+    constexpr int nR = N / R;
+    constexpr int tstep = N / (NS * R);
+    constexpr int MAXPT = RPCF_REGC / R;
+    const int tot = B * nR;
+    C v[MAXPT][R];
+#pragma unroll
+    for (int it = 0; it < MAXPT; it++) {
+        const int t = tid + it * nthr;
+        if (t < tot) {
+            const int b = t / nR, j = t - b * nR, k = j % NS;
+            const C* base = buf + b * ld;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                C x = base[j + r * nR];
+                if (r > 0 && NS > 1) {
+                    C w = tw[k * r * tstep];
+                    if (DIR > 0) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[it][r] = x;
+            }
+            Dft<R, DIR, C>::run(v[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MAXPT; it++) {
+        const int t = tid + it * nthr;
+        if (t < tot) {
+            const int b = t / nR, j = t - b * nR, k = j % NS;
+            C* base = buf + b * ld;
+            const int dst = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) base[dst + r * NS] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int DIR, typename C, int S = 0, int NS = 1>
+__device__ __forceinline__ void fft_ct(C* buf, int ld, int B, const C* tw, int tid, int nthr) {
+    if constexpr (S < ct_nst(N)) {
+        constexpr int R = ct_rad(N, S);
+        fft_stage_ct<R, DIR, N, NS>(buf, ld, B, tw, tid, nthr);
+        fft_ct<N, DIR, C, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
+    }
+}
+
+// FN > 0: the compile-time plan of length FN (must equal pl.n); FN = 0: the runtime plan.
+template <int FN, int DIR, typename C>
+__device__ __forceinline__ void fft_any(C* buf, int ld, int B, const FftPlan& pl, int tid, int nthr) {
+    if constexpr (FN > 0) fft_ct<FN, DIR>(buf, ld, B, (const C*)pl.tw, tid, nthr);
+    else fft_run<DIR>(buf, ld, B, pl, tid, nthr);
+}
+
+// lengths with compiled plans (every grid of the benchmark, ensemble and serving sets)
+#define RPCF_FFT_LENGTHS(X) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)

@@ -7,9 +7,11 @@
 // Element (r, j) of slot s of equation e sits at minv[(e*nslot + s)*nz*nz + j*nz + r],
 // so thread r of a solve reads consecutive addresses for every j.
 //
-// Arithmetic follows the reference expressions in the same order (the oracle restates
-// them identically), so the matrices and the linear update are reproducible bit for bit
-// from identical inputs when compiled without FP contraction.
+// The matrix build, the inverse, the mean-flow update and the velocity recovery follow
+// the reference expressions in the same order as the oracle (no FP contraction), so they
+// reproduce it bit for bit from identical inputs.  The per-mode update regroups the
+// right-hand-side sums (three dot products per row, fused multiply-adds) and splits the
+// solves over four waves; it matches the oracle to rounding (about 1e-16 relative).
 #pragma once
 #include "rpcf_common.hpp"
 
@@ -227,8 +229,23 @@ __global__ void k_velocity(const double2* __restrict__ sv, const double2* __rest
 // r-th entry of (minv of slot) * x, j ascending (m_multi_v, source/matrix_op.cpp:68-78)
 __device__ __forceinline__ double2 gemv_row(const double2* __restrict__ M, const double2* x, int r, int n) {
     double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 16
     for (int j = 0; j < n; j++) acc = zadd(acc, zmul(M[(long)j * n + r], x[j]));
     return acc;
+}
+
+// both pencils of a slot against one matrix: the matrix is read once
+__device__ __forceinline__ void gemv_row2(const double2* __restrict__ M, const double2* x0, const double2* x1, int r,
+                                          int n, double2* y0, double2* y1) {
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+#pragma unroll 16
+    for (int j = 0; j < n; j++) {
+        const double2 m = M[(long)j * n + r];
+        a0 = zadd(a0, zmul(m, x0[j]));
+        a1 = zadd(a1, zmul(m, x1[j]));
+    }
+    *y0 = a0;
+    *y1 = a1;
 }
 
 struct LinState {
@@ -243,146 +260,241 @@ struct LinState {
     int flow_control;
 };
 
+// Batched GEMV of the two pencils of a slot: y_q = M x_q, M column-major n x n.
+// Wave w of the block sums the columns j in its quarter of the range for the RPL rows
+// lane, lane+64, ... (coalesced 1 KB loads, RPL*J/4 of them independent per lane); the
+// four partial sums are added through LDS (part[4][2][nz]) in wave order.
+template <int RPL>
+__device__ __forceinline__ void gemv_slot(const double2* __restrict__ M, const double2* x0, const double2* x1, int n,
+                                          double2* part, double2* y0, double2* y1) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int jb = (n * w) / 4, je = (n * (w + 1)) / 4;
+    double2 a0[RPL], a1[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; i++) a0[i] = a1[i] = make_double2(0.0, 0.0);
+#pragma unroll 2
+    for (int j = jb; j < je; j++) {
+        const double2 u0 = x0[j], u1 = x1[j];
+        const double2* col = M + (long)j * n;
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            const int r = lane + 64 * i;
+            if (r < n) {
+                const double2 m = col[r];
+                a0[i].x = fma(m.x, u0.x, fma(-m.y, u0.y, a0[i].x));
+                a0[i].y = fma(m.x, u0.y, fma(m.y, u0.x, a0[i].y));
+                a1[i].x = fma(m.x, u1.x, fma(-m.y, u1.y, a1[i].x));
+                a1[i].y = fma(m.x, u1.y, fma(m.y, u1.x, a1[i].y));
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPL; i++) {
+        const int r = lane + 64 * i;
+        if (r < n) {
+            part[(w * 2 + 0) * n + r] = a0[i];
+            part[(w * 2 + 1) * n + r] = a1[i];
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        double2 s0 = part[r], s1 = part[n + r];
+        for (int q = 1; q < 4; q++) {
+            s0 = zadd(s0, part[(q * 2) * n + r]);
+            s1 = zadd(s1, part[(q * 2 + 1) * n + r]);
+        }
+        y0[r] = s0;
+        y1[r] = s1;
+    }
+    __syncthreads();
+}
+
+// Mean-flow equations of the (0,0) slot: _get_linear_u0/_w0 (source/linear.cpp:173-205),
+// their solves, and the constant-flow-rate correction of u0 (u0 += f u_h with u_h the
+// response to a unit uniform forcing and f restoring the initial bulk).  Runs as block 0
+// of k_linear so that it overlaps the per-mode blocks.
+template <int RPL>
+__device__ void linear_mean_block(const LinState& st, const Dims& d, const LinParams& lp, const double* __restrict__ T0t,
+                                  const double* __restrict__ T2t, const double* __restrict__ qw, double2* lds) {
+    const int n = d.nz, N = d.N, r = threadIdx.x;
+    const double Re = lp.Re, dt = lp.dt;
+    const long nn = (long)n * n;
+    double2* vn = lds;
+    double2* rhs0 = lds + n;
+    double2* rhs1 = lds + 2 * n;
+    double2* x0 = lds + 3 * n;
+    double2* x1 = lds + 4 * n;
+    double2* sf = lds + 5 * n;
+    double2* part = lds + 8 * n;
+    for (int e = 0; e < 2; e++) {
+        double2* s = e ? st.se : st.sv;
+        const double2* l0 = e ? st.nle : st.nlv;
+        const double2* l0p = e ? st.nlep : st.nlvp;
+        if (r < n) vn[r] = s[r];
+        __syncthreads();
+        if (r < n) {
+            double2 v = make_double2(0.0, 0.0);
+            if (r >= 2) {
+                double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 8
+                for (int j = 0; j <= N; j++) {
+                    const long i1 = (long)n * j + (r - 1);
+                    acc = zadd(acc, zmul(vn[j], make_double2(T0t[i1] + dt * 0.5 / Re * T2t[i1], 0.0)));
+                }
+                v = zsub(acc, zmulr(zsub(zmulr(l0[r - 1], 1.5), zmulr(l0p[r - 1], 0.5)), dt));
+            }
+            rhs0[r] = v;
+            rhs1[r] = make_double2(r >= 2 ? dt : 0.0, 0.0);  // unit uniform forcing
+        }
+        __syncthreads();
+        gemv_slot<RPL>(st.minv + (e ? (long)d.nslot * nn : 0), rhs0, rhs1, n, part, x0, x1);
+        if (e == 0 && st.flow_control == 1) {
+            if (r == 0) {
+                double2 qp = make_double2(0.0, 0.0);
+                double qh = 0.0;
+                for (int k = 0; k < n; k++) {
+                    qp = zadd(qp, zmulr(x0[k], qw[k]));
+                    qh = qh + x1[k].x * qw[k];
+                }
+                sf[0] = make_double2((st.qw_target_re - qp.x) / qh, (st.qw_target_im - qp.y) / qh);
+            }
+            __syncthreads();
+            if (r < n) x0[r] = zadd(x0[r], zmul(sf[0], x1[r]));
+        }
+        if (r < n) s[r] = x0[r];
+        __syncthreads();
+    }
+}
+
 // -------------------------------------------------------------------------------------
 // One Crank-Nicolson/AB2 update of both prognostic equations for the one or two pencils
-// sharing a matrix slot: _get_linear_v + AB2 (source/linear.cpp:86-125,
-// source/rhs.cpp:205-215), v solve (source/solver.cpp:107-123), _get_linear_omega_y
-// (source/linear.cpp:127-171), eta solve; the slot (0,0) runs the mean-flow equations
-// _get_linear_u0/_w0 (source/linear.cpp:173-205) and the constant-flow-rate correction.
-// One block per slot, thread r = row r.
+// sharing a matrix slot (every slot but (0,0)): _get_linear_v + AB2
+// (source/linear.cpp:86-125, source/rhs.cpp:205-215), v solve (source/solver.cpp:107-123),
+// _get_linear_omega_y (source/linear.cpp:127-171), eta solve.
+// The right-hand sides use the linearity of the reference's per-element operator,
+//   sum_j v_j B_rj = (T4q v)_p + c2 (T2 v)_p - c0 (T0 v)_p + i [a2_p (T2 v)_p + a0_p (T0 v)_p],
+// so each row needs three real dot products per pencil (fused multiply-adds) instead of
+// rebuilding B_rj; T0, T2 and the quotients T4q = T4*dt*0.5/Re, T2q = T2*0.5/Re*dt arrive
+// transposed (element (p, j) at j*nz + p: consecutive rows, consecutive addresses).
+// Block of 256 threads; rows r < nz map to threads.
 // -------------------------------------------------------------------------------------
-__global__ void k_linear(LinState st, Dims d, LinParams lp, const double* __restrict__ T0,
-                         const double* __restrict__ T2, const double* __restrict__ T4, const double* __restrict__ U,
-                         const double* __restrict__ dU, const double* __restrict__ ddU,
-                         const double* __restrict__ qw) {
-    __shared__ double2 vn[2][256], vnew[2][256], rhs[2][256], en[2][256];
+// LDS (dynamic, 16*nz*16 bytes): vn, en, rhs, vnew (two pencils each) and part[4][2][nz].
+// Block 0 runs the mean-flow equations of slot (0,0); block b > 0 runs slot b.
+template <int RPL>
+__global__ __launch_bounds__(256, 4) void k_linear(LinState st, Dims d, LinParams lp, const double* __restrict__ T0,
+                                                   const double* __restrict__ T2, const double* __restrict__ T4q,
+                                                   const double* __restrict__ T2q, const double* __restrict__ U,
+                                                   const double* __restrict__ dU, const double* __restrict__ ddU,
+                                                   const double* __restrict__ qw) {
+    extern __shared__ __align__(16) unsigned char lin_lds_raw[];
+    const int n = d.nz, N = d.N, r = threadIdx.x;
+    double2* lds = (double2*)lin_lds_raw;
+    double2* vn[2] = {lds, lds + n};
+    double2* en[2] = {lds + 2 * n, lds + 3 * n};
+    double2* rhs[2] = {lds + 4 * n, lds + 5 * n};
+    double2* vnew[2] = {lds + 6 * n, lds + 7 * n};
+    double2* part = lds + 8 * n;
+    if (blockIdx.x == 0) {
+        linear_mean_block<RPL>(st, d, lp, T0, T2, qw, lds);
+        return;
+    }
     const int slot = blockIdx.x;
     const int kyh = slot / d.hnx, kx = slot - kyh * d.hnx;
-    const int n = d.nz, N = d.N, r = threadIdx.x;
     const double Re = lp.Re, dt = lp.dt;
     const long nn = (long)n * n;
     const double2* Mv = st.minv + (long)slot * nn;
     const double2* Mo = st.minv + ((long)d.nslot + slot) * nn;
-
-    if (kx == 0 && kyh == 0) {
-        // mean flow: u0 with L_x(0,0), w0 with L_y(0,0) -- the (0,0) pencils of N_v, N_eta
-        for (int e = 0; e < 2; e++) {
-            double2* s = e ? st.se : st.sv;
-            const double2* l0 = e ? st.nle : st.nlv;
-            const double2* l0p = e ? st.nlep : st.nlvp;
-            const double2* M = e ? Mo : Mv;
-            if (r < n) vn[0][r] = s[r];
-            __syncthreads();
-            if (r < n) {
-                double2 v = make_double2(0.0, 0.0);
-                if (r >= 2) {
-                    double2 acc = make_double2(0.0, 0.0);
-                    for (int j = 0; j <= N; j++) {
-                        const long i1 = (long)n * (r - 1) + j;
-                        acc = zadd(acc, zmul(vn[0][j], make_double2(T0[i1] + dt * 0.5 / Re * T2[i1], 0.0)));
-                    }
-                    v = zsub(acc, zmulr(zsub(zmulr(l0[r - 1], 1.5), zmulr(l0p[r - 1], 0.5)), dt));
-                }
-                rhs[0][r] = v;
-                // unit-forcing response for the flow-rate control: (0, 0, dt, ..., dt)
-                rhs[1][r] = make_double2(r >= 2 ? dt : 0.0, 0.0);
-            }
-            __syncthreads();
-            double2 x = make_double2(0.0, 0.0), uh = make_double2(0.0, 0.0);
-            if (r < n) {
-                x = gemv_row(M, rhs[0], r, n);
-                uh = gemv_row(Mv, rhs[1], r, n);
-                vnew[0][r] = x;
-                vnew[1][r] = uh;
-            }
-            __syncthreads();
-            if (e == 0 && st.flow_control == 1) {
-                __shared__ double2 s_f;
-                if (r == 0) {
-                    double2 qp = make_double2(0.0, 0.0);
-                    double qh = 0.0;
-                    for (int k = 0; k < n; k++) {
-                        qp = zadd(qp, zmulr(vnew[0][k], qw[k]));
-                        qh = qh + vnew[1][k].x * qw[k];
-                    }
-                    s_f = make_double2((st.qw_target_re - qp.x) / qh, (st.qw_target_im - qp.y) / qh);
-                }
-                __syncthreads();
-                if (r < n) x = zadd(x, zmul(s_f, uh));
-            }
-            if (r < n) s[r] = x;
-            __syncthreads();
-        }
-        return;
-    }
-
-    // two pencils per slot: ky = kyh and ky = ny - kyh (same matrices); one if kyh is 0 or ny/2
-    const int np = (kyh == 0 || kyh == d.ny / 2) ? 1 : 2;
-    long po[2];
-    po[0] = ((long)kyh * d.hnx + kx) * n;
-    po[1] = ((long)(d.ny - kyh) * d.hnx + kx) * n;
-    double alpha, beta0;
+    // pencils ky = kyh and ky = ny - kyh share the slot; with kyh = 0 or ny/2 the slot has
+    // one pencil and the second lane repeats it without being written back
+    const bool two = !(kyh == 0 || kyh == d.ny / 2);
+    const long po0 = ((long)kyh * d.hnx + kx) * n;
+    const long po1 = two ? ((long)(d.ny - kyh) * d.hnx + kx) * n : po0;
+    double alpha, beta0, beta1, a_unused;
     ab_of(d, lp, kx, kyh, &alpha, &beta0);
+    ab_of(d, lp, kx, two ? d.ny - kyh : kyh, &a_unused, &beta1);
     const double kmn = alpha * alpha + beta0 * beta0;
-    double betas[2];
-    {
-        double a2, b2;
-        ab_of(d, lp, kx, d.ny - kyh, &a2, &b2);
-        betas[0] = beta0;
-        betas[1] = b2;
+    if (r < n) {
+        vn[0][r] = st.sv[po0 + r];
+        en[0][r] = st.se[po0 + r];
+        vn[1][r] = st.sv[po1 + r];
+        en[1][r] = st.se[po1 + r];
     }
-    if (r < n)
-        for (int q = 0; q < np; q++) {
-            vn[q][r] = st.sv[po[q] + r];
-            en[q][r] = st.se[po[q] + r];
-        }
     __syncthreads();
     // v right-hand side, rows 4..N at collocation point p = r-2; rows 0..3 (BCs) zero
     if (r < n) {
-        double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+        double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
         if (r >= 4) {
             const int pr = r - 2;
+            double s0r = 0, s0i = 0, s2r = 0, s2i = 0, s4r = 0, s4i = 0;  // pencil 0: T0 v, T2 v, T4q v
+            double t0r = 0, t0i = 0, t2r = 0, t2i = 0, t4r = 0, t4i = 0;  // pencil 1
+#pragma unroll 2
             for (int j = 0; j <= N; j++) {
-                const long i2 = (long)n * pr + j;
-                const double2 b = make_double2(
-                    T4[i2] * dt * 0.5 / Re + (1 - kmn * dt / Re) * T2[i2] - kmn * (1 - kmn * dt * 0.5 / Re) * T0[i2],
-                    -alpha * dt * 0.5 * U[pr] * T2[i2] +
-                        (kmn * alpha * dt * 0.5 * U[pr] + alpha * dt * 0.5 * ddU[pr]) * T0[i2]);
-                for (int q = 0; q < np; q++) acc[q] = zadd(acc[q], zmul(vn[q][j], b));
+                const long ij = (long)n * j + pr;
+                const double c0 = T0[ij], c2 = T2[ij], c4 = T4q[ij];
+                const double2 x = vn[0][j], y = vn[1][j];
+                s0r = fma(c0, x.x, s0r); s0i = fma(c0, x.y, s0i);
+                s2r = fma(c2, x.x, s2r); s2i = fma(c2, x.y, s2i);
+                s4r = fma(c4, x.x, s4r); s4i = fma(c4, x.y, s4i);
+                t0r = fma(c0, y.x, t0r); t0i = fma(c0, y.y, t0i);
+                t2r = fma(c2, y.x, t2r); t2i = fma(c2, y.y, t2i);
+                t4r = fma(c4, y.x, t4r); t4i = fma(c4, y.y, t4i);
             }
-            for (int q = 0; q < np; q++)
-                acc[q] = zadd(acc[q], zmulr(zsub(zmulr(st.nlv[po[q] + pr], 1.5), zmulr(st.nlvp[po[q] + pr], 0.5)), dt));
+            const double k2 = (1 - kmn * dt / Re), k0 = kmn * (1 - kmn * dt * 0.5 / Re);
+            const double i2 = -alpha * dt * 0.5 * U[pr];
+            const double i0 = (kmn * alpha * dt * 0.5 * U[pr] + alpha * dt * 0.5 * ddU[pr]);
+            // (B v) = (T4q v + k2 T2 v - k0 T0 v) + i (i2 T2 v + i0 T0 v), with complex v
+            const double br0 = s4r + k2 * s2r - k0 * s0r, bi0 = s4i + k2 * s2i - k0 * s0i;
+            const double br1 = t4r + k2 * t2r - k0 * t0r, bi1 = t4i + k2 * t2i - k0 * t0i;
+            a0 = make_double2(br0 - (i2 * s2i + i0 * s0i), bi0 + (i2 * s2r + i0 * s0r));
+            a1 = make_double2(br1 - (i2 * t2i + i0 * t0i), bi1 + (i2 * t2r + i0 * t0r));
+            a0 = zadd(a0, zmulr(zsub(zmulr(st.nlv[po0 + pr], 1.5), zmulr(st.nlvp[po0 + pr], 0.5)), dt));
+            a1 = zadd(a1, zmulr(zsub(zmulr(st.nlv[po1 + pr], 1.5), zmulr(st.nlvp[po1 + pr], 0.5)), dt));
         }
-        for (int q = 0; q < np; q++) rhs[q][r] = acc[q];
+        rhs[0][r] = a0;
+        rhs[1][r] = a1;
     }
     __syncthreads();
-    if (r < n)
-        for (int q = 0; q < np; q++) vnew[q][r] = gemv_row(Mv, rhs[q], r, n);
-    __syncthreads();
-    // eta right-hand side, rows 2..N at p = r-1; rows 0,1 zero
+    gemv_slot<RPL>(Mv, rhs[0], rhs[1], n, part, vnew[0], vnew[1]);
+    // eta right-hand side, rows 2..N at p = r-1; rows 0,1 zero:
+    //   sum_j eta_j C_rj + (v_j + vnew_j) E_rj with C = T2q + c0 T0 + i ci T0, E = i e T0
     if (r < n) {
-        double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+        double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
         if (r >= 2) {
             const int pr = r - 1;
+            double s0r = 0, s0i = 0, s2r = 0, s2i = 0, w0r = 0, w0i = 0;  // pencil 0: T0 eta, T2q eta, T0 (v+vnew)
+            double t0r = 0, t0i = 0, t2r = 0, t2i = 0, z0r = 0, z0i = 0;  // pencil 1
+#pragma unroll 2
             for (int j = 0; j <= N; j++) {
-                const long i1 = (long)n * pr + j;
-                const double2 c = make_double2(T2[i1] * 0.5 / Re * dt + (1 - kmn * dt * 0.5 / Re) * T0[i1],
-                                               -alpha * dt * 0.5 * U[pr] * T0[i1]);
-                for (int q = 0; q < np; q++) {
-                    acc[q] = zadd(acc[q], zmul(en[q][j], c));
-                    acc[q] = zadd(acc[q], zmul(zadd(vn[q][j], vnew[q][j]),
-                                               make_double2(0.0, -0.5 * betas[q] * dU[pr] * dt * T0[i1])));
-                }
+                const long ij = (long)n * j + pr;
+                const double c0 = T0[ij], c2 = T2q[ij];
+                const double2 x = en[0][j], y = en[1][j];
+                const double2 vx = zadd(vn[0][j], vnew[0][j]), vy = zadd(vn[1][j], vnew[1][j]);
+                s0r = fma(c0, x.x, s0r); s0i = fma(c0, x.y, s0i);
+                s2r = fma(c2, x.x, s2r); s2i = fma(c2, x.y, s2i);
+                w0r = fma(c0, vx.x, w0r); w0i = fma(c0, vx.y, w0i);
+                t0r = fma(c0, y.x, t0r); t0i = fma(c0, y.y, t0i);
+                t2r = fma(c2, y.x, t2r); t2i = fma(c2, y.y, t2i);
+                z0r = fma(c0, vy.x, z0r); z0i = fma(c0, vy.y, z0i);
             }
-            for (int q = 0; q < np; q++)
-                acc[q] = zadd(acc[q], zmulr(zsub(zmulr(st.nle[po[q] + pr], 1.5), zmulr(st.nlep[po[q] + pr], 0.5)), dt));
+            const double c0k = (1 - kmn * dt * 0.5 / Re), ci = -alpha * dt * 0.5 * U[pr];
+            const double e0 = -0.5 * beta0 * dU[pr] * dt, e1 = -0.5 * beta1 * dU[pr] * dt;
+            // C eta = (T2q eta + c0k T0 eta) + i ci T0 eta ; E w = i e T0 w
+            a0 = make_double2(s2r + c0k * s0r - ci * s0i - e0 * w0i, s2i + c0k * s0i + ci * s0r + e0 * w0r);
+            a1 = make_double2(t2r + c0k * t0r - ci * t0i - e1 * z0i, t2i + c0k * t0i + ci * t0r + e1 * z0r);
+            a0 = zadd(a0, zmulr(zsub(zmulr(st.nle[po0 + pr], 1.5), zmulr(st.nlep[po0 + pr], 0.5)), dt));
+            a1 = zadd(a1, zmulr(zsub(zmulr(st.nle[po1 + pr], 1.5), zmulr(st.nlep[po1 + pr], 0.5)), dt));
         }
-        for (int q = 0; q < np; q++) rhs[q][r] = acc[q];
+        rhs[0][r] = a0;
+        rhs[1][r] = a1;
     }
     __syncthreads();
-    if (r < n)
-        for (int q = 0; q < np; q++) {
-            st.se[po[q] + r] = gemv_row(Mo, rhs[q], r, n);
-            st.sv[po[q] + r] = vnew[q][r];
+    gemv_slot<RPL>(Mo, rhs[0], rhs[1], n, part, en[0], en[1]);
+    if (r < n) {
+        st.se[po0 + r] = en[0][r];
+        st.sv[po0 + r] = vnew[0][r];
+        if (two) {
+            st.se[po1 + r] = en[1][r];
+            st.sv[po1 + r] = vnew[1][r];
         }
+    }
 }

@@ -20,44 +20,68 @@
 #define RPCF_LDS(T) extern __shared__ __align__(16) unsigned char rpcf_lds_raw[]; T* lds = (T*)rpcf_lds_raw
 
 // -------------------------------------------------------------------------------------
-// Chebyshev spectral -> physical along the wall-normal direction.
-// PAD=1: cheby_pre_s2p_pad + length-4N FFT (source/transform.cu:416-453, 679-698),
-//        values at the pz points cos(pi q/(pz-1));
-// PAD=0: cheby_pre_s2p_noPad + length-2N FFT (source/transform.cu:455-491, 716-733),
-//        values at the nz points cos(pi p/N).
-// Tile: TB pencils (kx fixed, ky0..ky0+TB); output Y[f][kx][q][ky].
+// Chebyshev spectral -> physical along the wall-normal direction, f(z_q) = sum_k a_k T_k(z_q).
+// PAD=0: the nz points cos(pi p/N): cheby_pre_s2p_noPad + length-2N FFT of the even
+//        extension (source/transform.cu:455-491, 716-733).
+// PAD=1: the pz = 2N+1 points cos(pi q/(2N)), which the reference takes from a length-4N
+//        FFT of the zero-padded even extension (source/transform.cu:416-453, 679-698).  Here
+//        the even points q = 2m are the PAD=0 transform of the same coefficients, and the odd
+//        points q = 2m+1 are g_m = sum_k a_k cos(pi k (2m+1)/(2N)) = FFT_2N(d)_m with
+//        d_0 = a_0, d_k = a_k/2 e^{-i pi k/(2N)}, d_{2N-k} = a_k/2 e^{+i pi k/(2N)} (0<k<N),
+//        d_N = 0 -- two length-2N FFTs in a 2N-point LDS tile instead of one of length 4N.
+// tw4: exp(-2 pi i k/(4N)), k < 4N.  Tile: TB pencils (kx fixed); output Y[f][kx][q][ky].
 // -------------------------------------------------------------------------------------
-template <int PAD>
-__global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F, long fstride, double2* __restrict__ Y, long ystride,
-                           Dims d, FftPlan pl, int TB, int ld) {
+__device__ __forceinline__ void s2p_even_ext(double2* s, double2 a, int k, int N) {
+    const int L = 2 * N;
+    if (k < N) {
+        const double2 t = cscale(a, 0.5);
+        if (k == 0) s[0] = cscale(t, 2.0);
+        else { s[k] = t; s[L - k] = t; }
+    } else {
+        s[N] = a;
+    }
+}
+
+__device__ __forceinline__ void s2p_odd_ext(double2* s, double2 a, int k, int N, const double2* tw4) {
+    const int L = 2 * N;
+    if (k == 0) s[0] = a;
+    else if (k < N) {
+        const double2 t = cscale(a, 0.5), w = tw4[k];
+        s[k] = cmul(t, w);
+        s[L - k] = cmul(t, make_double2(w.x, -w.y));
+    } else {
+        s[N] = make_double2(0.0, 0.0);
+    }
+}
+
+template <int PAD, int FN = 0>  // FN: compile-time 2N or 0
+__global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F, long fstride, double2* __restrict__ Y,
+                                                  long ystride, Dims d, FftPlan pl, const double2* __restrict__ tw4,
+                                                  int TB, int ld) {
     RPCF_LDS(double2);
     const int f = blockIdx.z, kx = blockIdx.y, ky0 = blockIdx.x * TB;
     const int nb = min(TB, d.ny - ky0);
-    const int N = d.N, nz = d.nz, L = PAD ? 4 * N : 2 * N, nout = PAD ? d.pz : d.nz;
+    const int N = d.N, nz = d.nz, nout = PAD ? d.pz : d.nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const double2* in = F + f * fstride;
-    if (PAD) {
-        for (int i = tid; i < nb * ld; i += nthr) lds[i] = make_double2(0.0, 0.0);
-        __syncthreads();
-    }
-    for (int i = tid; i < nb * nz; i += nthr) {
-        const int b = i / nz, k = i - b * nz;
-        const double2 a = in[((long)(ky0 + b) * d.hnx + kx) * nz + k];
-        double2* s = lds + b * ld;
-        if (PAD || k < N) {
-            double2 t = cscale(a, 0.5);
-            if (k == 0) s[0] = cscale(t, 2.0);
-            else { s[k] = t; s[L - k] = t; }
-        } else {
-            s[N] = a;
+    const double2* in = F + f * fstride + ((long)ky0 * d.hnx + kx) * nz;
+    double2* out = Y + f * ystride + (long)kx * nout * d.ny + ky0;
+    for (int half = 0; half < (PAD ? 2 : 1); half++) {
+        for (int i = tid; i < nb * nz; i += nthr) {
+            const int b = i / nz, k = i - b * nz;
+            const double2 a = in[(long)b * d.hnx * nz + k];
+            if (half == 0) s2p_even_ext(lds + b * ld, a, k, N);
+            else s2p_odd_ext(lds + b * ld, a, k, N, tw4);
         }
-    }
-    __syncthreads();
-    fft_run<-1>(lds, ld, nb, pl, tid, nthr);
-    double2* out = Y + f * ystride;
-    for (int i = tid; i < nout * nb; i += nthr) {
-        const int q = i / nb, b = i - q * nb;
-        out[((long)kx * nout + q) * d.ny + ky0 + b] = lds[b * ld + q];
+        __syncthreads();
+        fft_any<FN, -1>(lds, ld, nb, pl, tid, nthr);
+        // even half: points q = m (PAD=0) or q = 2m (PAD=1), m = 0..N; odd half: q = 2m+1, m < N
+        const int cnt = half ? N : N + 1;
+        for (int i = tid; i < cnt * nb; i += nthr) {
+            const int m = i / nb, b = i - m * nb;
+            const int q = PAD ? 2 * m + half : m;
+            out[(long)q * d.ny + b] = lds[b * ld + m];
+        }
+        __syncthreads();
     }
 }
 
@@ -67,6 +91,7 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
 // (source/transpose.cu:358-381, source/transform.cu:355-386).  Tile: TK kx columns of one
 // plane.  Y[f][kx][q][ky] -> W[f][q][s][kx] for kx < nxh (the x-Nyquist column is zero).
 // -------------------------------------------------------------------------------------
+template <int FN = 0>
 __global__ __launch_bounds__(256) void k_span_inv(const double2* __restrict__ Y, long ystride, double2* __restrict__ W, long wstride, Dims d,
                            int planes, FftPlan pl, int TK, int ld) {
     RPCF_LDS(double2);
@@ -80,7 +105,7 @@ __global__ __launch_bounds__(256) void k_span_inv(const double2* __restrict__ Y,
         lds[kxi * ld + s] = ky >= 0 ? in[((long)(kx0 + kxi) * planes + q) * d.ny + ky] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_run<+1>(lds, ld, nk, pl, tid, nthr);
+    fft_any<FN, +1>(lds, ld, nk, pl, tid, nthr);
     double2* out = W + f * wstride;
     for (int i = tid; i < my * nk; i += nthr) {
         const int s = i / nk, kxi = i - s * nk;
@@ -119,6 +144,7 @@ __device__ __forceinline__ void r2c_unpack(const double2* z, int k, int mx, doub
 // 1/(mx my) normalisation of source/transform.cu:267.  One block: one plane q, RP
 // pairs of span rows.  W[6][pz][my][nxh] -> Z2[3][pz][my][hnx].
 // -------------------------------------------------------------------------------------
+template <int FN = 0>
 __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W, long wstride, double2* __restrict__ Z2, long zstride,
                             const double* __restrict__ meanU, double Ro, double fac, Dims d, FftPlan pl, int RP,
                             int ld) {
@@ -133,7 +159,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W
         lds[seq * ld + k] = c2r_pack(rA, rA + nxh, k, mx, nxh);
     }
     __syncthreads();
-    fft_run<+1>(lds, ld, 6 * nrp, pl, tid, nthr);
+    fft_any<FN, +1>(lds, ld, 6 * nrp, pl, tid, nthr);
     const double mU = meanU[q];
     for (int i = tid; i < nrp * mx; i += nthr) {
         const int ip = i / mx, x = i - ip * mx;
@@ -148,7 +174,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W
         lds[(2 * nrp + ip) * ld + x] = make_double2(lza, lzb);
     }
     __syncthreads();
-    fft_run<-1>(lds, ld, 3 * nrp, pl, tid, nthr);
+    fft_any<FN, -1>(lds, ld, 3 * nrp, pl, tid, nthr);
     const int hnx = d.hnx;
     for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
         const int seq = i / hnx, k = i - seq * hnx;
@@ -213,6 +239,7 @@ __global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, 
 // reference's D2Z (source/transform.cu:240) and transpose_forward
 // (source/transpose.cu:223-257).  Z2[g][q][s][kx] -> Z1[g][kx][q][ky].
 // -------------------------------------------------------------------------------------
+template <int FN = 0>
 __global__ __launch_bounds__(256) void k_span_fwd(const double2* __restrict__ Z2, long zstride, double2* __restrict__ Z1, long ostride, Dims d,
                            int planes, FftPlan pl, int TK, int ld) {
     RPCF_LDS(double2);
@@ -225,7 +252,7 @@ __global__ __launch_bounds__(256) void k_span_fwd(const double2* __restrict__ Z2
         lds[kxi * ld + s] = in[((long)q * my + s) * hnx + kx0 + kxi];
     }
     __syncthreads();
-    fft_run<-1>(lds, ld, nk, pl, tid, nthr);
+    fft_any<FN, -1>(lds, ld, nk, pl, tid, nthr);
     double2* out = Z1 + g * ostride;
     for (int i = tid; i < nk * d.ny; i += nthr) {
         const int kxi = i / d.ny, ky = i - kxi * d.ny;
@@ -283,30 +310,61 @@ __device__ __forceinline__ double2 ddz_at(const double2* a, int k, int n) {
 // (source/nonlinear.cu:146-242) and the unpadded s2p of N_v and N_eta
 // (source/rhs.cpp:96-97).  Tile: TB pencils (kx fixed).  Z1[3][hnx][pz][ny] ->
 // NLv, NLe [P][nz] (values at the nz collocation points; pencil (0,0) keeps L_x, L_y).
-// LDS: FFT buffer TB*ld, then 4 coefficient arrays TB*nz (L_x, L_y, L_z/N_v, N_eta).
+//
+// The reference's p2s is a length-4N FFT of the even extension u of the pz values
+// (cheby_pre_p2s/cheby_post_p2s, source/transform.cu:494-536).  Split by the parity of
+// the sample index (one radix-2 decimation-in-time step):
+//   F_k = E_k + w4^k H_k,  E = FFT_2N(even extension of f_0, f_2, .., f_2N),
+//                          H = FFT_2N(f_1, f_3, .., f_{2N-1}, f_{2N-1}, .., f_1),
+// w4 = exp(-2 pi i/(4N)); a_k = F_k/(2N), a_0 halved; only k < nz is kept.
+// LDS: FFT tile TB*ld (ld >= 2N+1), then coefficient arrays Lx, Ly, Lz [TB][nz].
 // -------------------------------------------------------------------------------------
+template <int FN = 0>  // FN: compile-time 2N or 0
 __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, long zstride, double2* __restrict__ NLv,
-                         double2* __restrict__ NLe, Dims d, FftPlan plP, FftPlan plN, double aphi, double beta,
-                         int TB, int ld) {
+                                                double2* __restrict__ NLe, Dims d, FftPlan plN,
+                                                const double2* __restrict__ tw4, double aphi, double beta, int TB,
+                                                int ld) {
     RPCF_LDS(double2);
     const int kx = blockIdx.y, ky0 = blockIdx.x * TB;
-    const int nb = min(TB, d.ny - ky0), nz = d.nz, N = d.N, pz = d.pz;
+    const int nb = min(TB, d.ny - ky0), nz = d.nz, N = d.N, ny = d.ny;
+    const int L = 2 * N;
     const int tid = threadIdx.x, nthr = blockDim.x;
     double2* fb = lds;
-    double2* coef = lds + TB * ld;  // [4][TB][nz]
-    const double factor = (1.0 / (pz - 1));
+    double2* coef = lds + TB * ld;  // [3][TB][nz]
+    const double factor = (1.0 / (d.pz - 1));
     for (int g = 0; g < 3; g++) {
-        load_p2s(fb, ld, Z1 + g * zstride, kx, ky0, nb, pz, d.ny, tid, nthr);
-        __syncthreads();
-        fft_run<-1>(fb, ld, nb, plP, tid, nthr);
-        for (int i = tid; i < nb * nz; i += nthr) {
-            const int b = i / nz, k = i - b * nz;
-            double2 t = cscale(fb[b * ld + k], factor);
-            if (k == 0) t = cscale(t, 0.5);
-            coef[(g * TB + b) * nz + k] = t;
+        const double2* in = Z1 + g * zstride + (long)kx * d.pz * ny + ky0;
+        for (int half = 0; half < 2; half++) {
+            const int cnt = half ? N : N + 1;  // samples f_{2m+half}
+            for (int i = tid; i < cnt * nb; i += nthr) {
+                const int m = i / nb, b = i - m * nb;
+                const double2 v = in[(long)(2 * m + half) * ny + b];
+                double2* sq = fb + b * ld;
+                if (half == 0) {
+                    sq[m] = v;
+                    if (m >= 1 && m <= N - 1) sq[L - m] = v;
+                } else {
+                    sq[m] = v;
+                    sq[L - 1 - m] = v;
+                }
+            }
+            __syncthreads();
+            fft_any<FN, -1>(fb, ld, nb, plN, tid, nthr);
+            for (int i = tid; i < nb * nz; i += nthr) {
+                const int b = i / nz, k = i - b * nz;
+                double2* c = coef + (g * TB + b) * nz + k;
+                if (half == 0) {
+                    *c = fb[b * ld + k];
+                } else {
+                    double2 t = cscale(cadd(*c, cmul(tw4[k], fb[b * ld + k])), factor);
+                    if (k == 0) t = cscale(t, 0.5);
+                    *c = t;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
+    // rhsNonlinear: N_v -> the Lz slot (read only by its own thread), N_eta -> the FFT tile
     for (int i = tid; i < nb * nz; i += nthr) {
         const int b = i / nz, k = i - b * nz, ky = ky0 + b;
         const double2* Lx = coef + (0 * TB + b) * nz;
@@ -318,7 +376,7 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, 
             ne = Ly[k];
         } else {
             const double ia = (double)kx / aphi;
-            const double ib = ky >= d.ny / 2 + 1 ? (double)(ky - d.ny) / beta : (double)ky / beta;
+            const double ib = ky >= ny / 2 + 1 ? (double)(ky - ny) / beta : (double)ky / beta;
             const double kmn = ia * ia + ib * ib;
             const double2 dx = ddz_at(Lx, k, nz), dy = ddz_at(Ly, k, nz);
             double ure = kmn * Lz[k].x - ia * dx.y;
@@ -331,26 +389,22 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, 
             ne = make_double2(wre, wim);
         }
         Lz[k] = nv;
-        coef[(3 * TB + b) * nz + k] = ne;
+        fb[b * ld + k] = ne;
     }
     __syncthreads();
-    const int L2 = 2 * N;
+    for (int i = tid; i < nb * nz; i += nthr) {
+        const int b = i / nz, k = i - b * nz;
+        coef[(0 * TB + b) * nz + k] = fb[b * ld + k];  // N_eta -> the Lx slot
+    }
+    __syncthreads();
     for (int h = 0; h < 2; h++) {
-        const double2* src = coef + (2 + h) * TB * nz;
+        const double2* src = coef + (h ? 0 : 2) * TB * nz;  // N_v in the Lz slot, N_eta in the Lx slot
         for (int i = tid; i < nb * nz; i += nthr) {
             const int b = i / nz, k = i - b * nz;
-            const double2 a = src[b * nz + k];
-            double2* s = fb + b * ld;
-            if (k < N) {
-                const double2 t = cscale(a, 0.5);
-                if (k == 0) s[0] = cscale(t, 2.0);
-                else { s[k] = t; s[L2 - k] = t; }
-            } else {
-                s[N] = a;
-            }
+            s2p_even_ext(fb + b * ld, src[b * nz + k], k, N);
         }
         __syncthreads();
-        fft_run<-1>(fb, ld, nb, plN, tid, nthr);
+        fft_any<FN, -1>(fb, ld, nb, plN, tid, nthr);
         double2* out = h ? NLe : NLv;
         for (int i = tid; i < nb * nz; i += nthr) {
             const int b = i / nz, p = i - b * nz;

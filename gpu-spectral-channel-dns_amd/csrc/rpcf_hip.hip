@@ -41,7 +41,7 @@ struct rpcf_ctx {
     int lds_limit = 65536;
     // tiles: sequences per block and LDS row stride (elements) of each transform kernel
     int TB_s2p = 1, TB_nl = 1, TK_inv = 1, TK_fwd = 1, RP = 1;
-    int ld_zp = 0, ld_my = 0, ld_mx = 0;
+    int ld_zn = 0, ld_my = 0, ld_mx = 0;
     FftPlan pl_x{}, pl_y{}, pl_zp{}, pl_zn{};
     int cap_x = 0, cap_y = 0, cap_zp = 0, cap_zn = 0;  // fft_capacity of each plan
     // device memory
@@ -54,6 +54,8 @@ struct rpcf_ctx {
     double2* Z2 = nullptr;  // [3][pz][my][hnx]
     double2* minv = nullptr;
     double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *U = nullptr, *dU = nullptr, *ddU = nullptr;
+    // transposed tables for k_linear: T0, T2, and the quotients T4*dt*0.5/Re, T2*0.5/Re*dt
+    double *T0t = nullptr, *T2t = nullptr, *T4q = nullptr, *T2q = nullptr;
     double *meanU = nullptr, *qw = nullptr, *tw = nullptr;
     std::vector<double> qw_host;
     double2 q_target = make_double2(0.0, 0.0);
@@ -225,12 +227,18 @@ int enq_nonlinear(rpcf_ctx* c) {
     int rc;
     const long fS = (long)d.P * d.nz, yS = (long)d.hnx * d.pz * d.ny, wS = (long)d.pz * d.my * d.nxh,
                zS = (long)d.pz * d.my * d.hnx;
+    const double2* tw4 = (const double2*)c->pl_zp.tw;
     {
-        const int TB = c->TB_s2p, ld = c->ld_zp;
+        const int TB = c->TB_s2p, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, 6);
         size_t lds = sizeof(double2) * TB * ld;
         rc = launch(c, KID_S2P, [&] {
-            k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->Y, yS, d, c->pl_zp, TB, ld);
+            switch (2 * d.N) {
+#define RPCF_CASE(L) case L: k_cheb_s2p<1, L><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->Y, yS, d, c->pl_zn, tw4, TB, ld); break;
+                RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                default: k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->Y, yS, d, c->pl_zn, tw4, TB, ld);
+            }
         });
         if (rc) return rc;
     }
@@ -239,7 +247,12 @@ int enq_nonlinear(rpcf_ctx* c) {
         dim3 g((d.nxh + TK - 1) / TK, d.pz, 6);
         size_t lds = sizeof(double2) * TK * ld;
         rc = launch(c, KID_SPAN_INV, [&] {
-            k_span_inv<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->W, wS, d, d.pz, c->pl_y, TK, ld);
+            switch (d.my) {
+#define RPCF_CASE(L) case L: k_span_inv<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->W, wS, d, d.pz, c->pl_y, TK, ld); break;
+                RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                default: k_span_inv<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->W, wS, d, d.pz, c->pl_y, TK, ld);
+            }
         });
         if (rc) return rc;
     }
@@ -248,8 +261,13 @@ int enq_nonlinear(rpcf_ctx* c) {
         dim3 g((d.my / 2 + RP - 1) / RP, d.pz);
         size_t lds = sizeof(double2) * 6 * RP * ld;
         rc = launch(c, KID_LAMB, [&] {
-            k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->meanU, c->Ro, c->fac, d, c->pl_x,
-                                                            RP, ld);
+            switch (d.mx) {
+#define RPCF_CASE(L) case L: k_lamb_phys<L><<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->meanU, c->Ro, c->fac, d, c->pl_x, RP, ld); break;
+                RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                default: k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->meanU, c->Ro, c->fac, d,
+                                                                         c->pl_x, RP, ld);
+            }
         });
         if (rc) return rc;
     }
@@ -258,7 +276,12 @@ int enq_nonlinear(rpcf_ctx* c) {
         dim3 g((d.hnx + TK - 1) / TK, d.pz, 3);
         size_t lds = sizeof(double2) * TK * ld;
         rc = launch(c, KID_SPAN_FWD, [&] {
-            k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->Y, yS, d, d.pz, c->pl_y, TK, ld);
+            switch (d.my) {
+#define RPCF_CASE(L) case L: k_span_fwd<L><<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->Y, yS, d, d.pz, c->pl_y, TK, ld); break;
+                RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                default: k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->Y, yS, d, d.pz, c->pl_y, TK, ld);
+            }
         });
         if (rc) return rc;
     }
@@ -266,12 +289,17 @@ int enq_nonlinear(rpcf_ctx* c) {
     std::swap(c->nl[0], c->nl[1]);
     std::swap(c->nl[2], c->nl[3]);
     {
-        const int TB = c->TB_nl, ld = c->ld_zp;
+        const int TB = c->TB_nl, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx);
-        size_t lds = sizeof(double2) * ((size_t)TB * ld + 4 * (size_t)TB * d.nz);
+        size_t lds = sizeof(double2) * ((size_t)TB * ld + 3 * (size_t)TB * d.nz);
         rc = launch(c, KID_P2S_NL, [&] {
-            k_p2s_nl<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->nl[0], c->nl[2], d, c->pl_zp, c->pl_zn,
-                                                         c->lp.aphi, c->lp.beta, TB, ld);
+            switch (2 * d.N) {
+#define RPCF_CASE(L) case L: k_p2s_nl<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->nl[0], c->nl[2], d, c->pl_zn, tw4, c->lp.aphi, c->lp.beta, TB, ld); break;
+                RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                default: k_p2s_nl<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->nl[0], c->nl[2], d, c->pl_zn, tw4,
+                                                                      c->lp.aphi, c->lp.beta, TB, ld);
+            }
         });
     }
     return rc;
@@ -284,9 +312,17 @@ int enq_linear(rpcf_ctx* c) {
     st.sv = c->sv; st.se = c->se; st.minv = c->minv;
     st.qw_target_re = c->q_target.x; st.qw_target_im = c->q_target.y;
     st.flow_control = c->prm.flow_control;
-    const int thr = 64 * ((d.nz + 63) / 64);
     return launch(c, KID_LINEAR, [&] {
-        k_linear<<<d.nslot, thr, 0, c->stream>>>(st, d, c->lp, c->T0, c->T2, c->T4, c->U, c->dU, c->ddU, c->qw);
+        const int g = d.nslot;
+        const size_t lds = sizeof(double2) * 16 * (size_t)d.nz;
+#define RPCF_LIN(R) k_linear<R><<<g, 256, lds, c->stream>>>(st, d, c->lp, c->T0t, c->T2t, c->T4q, c->T2q, c->U, c->dU, c->ddU, c->qw)
+        switch ((d.nz + 63) / 64) {
+            case 1: RPCF_LIN(1); break;
+            case 2: RPCF_LIN(2); break;
+            case 3: RPCF_LIN(3); break;
+            default: RPCF_LIN(4); break;
+        }
+#undef RPCF_LIN
     });
 }
 
@@ -294,7 +330,7 @@ void free_ctx(rpcf_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->F, c->sv, c->se, c->nl[0], c->nl[1], c->nl[2], c->nl[3], c->Y, c->W, c->Z2, c->minv,
-                    c->T0, c->T2, c->T4, c->U, c->dU, c->ddU, c->meanU, c->qw, c->tw};
+                    c->T0, c->T2, c->T4, c->U, c->dU, c->ddU, c->meanU, c->qw, c->tw, c->T0t, c->T2t, c->T4q, c->T2q};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -367,6 +403,9 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     if ((rc = dalloc(c, &c->minv, 2 * (size_t)d.nslot * nz * nz))) return rc;
     if ((rc = dalloc(c, &c->T0, nz * nz)) || (rc = dalloc(c, &c->T2, nz * nz)) || (rc = dalloc(c, &c->T4, nz * nz)))
         return rc;
+    if ((rc = dalloc(c, &c->T0t, nz * nz)) || (rc = dalloc(c, &c->T2t, nz * nz)) || (rc = dalloc(c, &c->T4q, nz * nz)) ||
+        (rc = dalloc(c, &c->T2q, nz * nz)))
+        return rc;
     if ((rc = dalloc(c, &c->U, nz)) || (rc = dalloc(c, &c->dU, nz)) || (rc = dalloc(c, &c->ddU, nz)) ||
         (rc = dalloc(c, &c->qw, nz)) || (rc = dalloc(c, &c->meanU, (size_t)d.pz)))
         return rc;
@@ -386,6 +425,23 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     HIPCHK(c, hipMemcpy(c->T0, T0.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->T2, T2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->T4, T4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    {
+        // transposed T0, and the quotients T2*0.5/Re*dt and T4*dt*0.5/Re of
+        // source/linear.cpp:100, 142 evaluated in the reference's order
+        std::vector<double> t0(nz * nz), t2(nz * nz), q4(nz * nz), q2(nz * nz);
+        const double Re = prm->Re, dt = prm->dt;
+        for (size_t i = 0; i < nz; i++)
+            for (size_t j = 0; j < nz; j++) {
+                t0[j * nz + i] = T0[i * nz + j];
+                t2[j * nz + i] = T2[i * nz + j];
+                q2[j * nz + i] = T2[i * nz + j] * 0.5 / Re * dt;
+                q4[j * nz + i] = T4[i * nz + j] * dt * 0.5 / Re;
+            }
+        HIPCHK(c, hipMemcpy(c->T0t, t0.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->T2t, t2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->T4q, q4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->T2q, q2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMemcpy(c->U, U.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dU, dU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->ddU, ddU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
@@ -404,14 +460,15 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
 
     // LDS tiles
     lim = c->lds_limit;
-    c->ld_zp = 4 * d.N + 1;
+    // Chebyshev kernels: tiles sized for three resident blocks per CU
+    const int lim3 = lim / 3;
+    c->ld_zn = 2 * d.N + 1;
     c->ld_my = d.my + 1;
     c->ld_mx = d.mx + 1;
-    c->TB_s2p = pick_tile(16, 4 * d.N, c->cap_zp, lim, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_zp; });
-    c->TB_nl = pick_tile(16, 4 * d.N, c->cap_zp, lim, [&](int t) {
-        return sizeof(double2) * ((size_t)t * c->ld_zp + 4 * (size_t)t * d.nz);
+    c->TB_s2p = pick_tile(16, 2 * d.N, c->cap_zn, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_zn; });
+    c->TB_nl = pick_tile(16, 2 * d.N, c->cap_zn, lim3, [&](int t) {
+        return sizeof(double2) * ((size_t)t * c->ld_zn + 3 * (size_t)t * d.nz);
     });
-    while (c->TB_nl > 1 && (long)c->TB_nl * 2 * d.N > (long)c->cap_zn) c->TB_nl--;
     c->TK_inv = pick_tile(16, d.my, c->cap_y, lim, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_my; });
     c->TK_fwd = c->TK_inv;
     c->RP = pick_tile(2, 6 * d.mx, c->cap_x, lim, [&](int t) { return sizeof(double2) * 6 * (size_t)t * c->ld_mx; });
@@ -419,10 +476,19 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
         return fail(c, RPCF_EINVAL, "grid too large for the LDS tiles (limit %d bytes)", lim);
     allow_lds(k_cheb_s2p<1>, lim);
     allow_lds(k_cheb_s2p<0>, lim);
-    allow_lds(k_span_inv, lim);
-    allow_lds(k_span_fwd, lim);
-    allow_lds(k_lamb_phys, lim);
-    allow_lds(k_p2s_nl, lim);
+    allow_lds(k_span_inv<0>, lim);
+    allow_lds(k_span_fwd<0>, lim);
+    allow_lds(k_lamb_phys<0>, lim);
+    allow_lds(k_p2s_nl<0>, lim);
+#define RPCF_CASE(L)                  \
+    allow_lds(k_cheb_s2p<1, L>, lim); \
+    allow_lds(k_cheb_s2p<0, L>, lim); \
+    allow_lds(k_span_inv<L>, lim);    \
+    allow_lds(k_span_fwd<L>, lim);    \
+    allow_lds(k_lamb_phys<L>, lim);   \
+    allow_lds(k_p2s_nl<L>, lim);
+    RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
     allow_lds(k_cheb_p2s, lim);
     allow_lds(k_x_c2r, lim);
     allow_lds(k_x_r2c, lim);
@@ -523,6 +589,12 @@ int rpcf_get_nonlinear(rpcf_ctx* c, int prev, double* n_v, double* n_eta, long s
     return rc ? rc : download_spec(c, n_eta, c->nl[prev ? 3 : 2], stride);
 }
 
+int rpcf_set_nonlinear(rpcf_ctx* c, int prev, const double* n_v, const double* n_eta, long stride) {
+    if (!c) return RPCF_EINVAL;
+    int rc = upload_spec(c, c->nl[prev ? 1 : 0], n_v, stride);
+    return rc ? rc : upload_spec(c, c->nl[prev ? 3 : 2], n_eta, stride);
+}
+
 int rpcf_nonlinear(rpcf_ctx* c) {
     if (!c) return RPCF_EINVAL;
     if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_nonlinear: no state set");
@@ -569,11 +641,11 @@ int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride
     if (!spec || !phys) return fail(c, RPCF_EINVAL, "rpcf_spec_to_phys: null array");
     const Dims d = c->d;
     const int planes = padded ? d.pz : d.nz;
-    const FftPlan& plz = padded ? c->pl_zp : c->pl_zn;
-    const int capz = padded ? c->cap_zp : c->cap_zn;
-    const int L = padded ? 4 * d.N : 2 * d.N, ld = L + 1;
-    const int TB = pick_tile(16, L, capz, c->lds_limit, [&](int t) { return sizeof(double2) * (size_t)t * ld; });
+    const FftPlan& plz = c->pl_zn;
+    const int L = 2 * d.N, ld = L + 1;
+    const int TB = pick_tile(16, L, c->cap_zn, c->lds_limit, [&](int t) { return sizeof(double2) * (size_t)t * ld; });
     const int RP1 = pick_tile(8, d.mx, c->cap_x, c->lds_limit, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_mx; });
+    const double2* tw4 = (const double2*)c->pl_zp.tw;
     double2 *f = nullptr, *y = nullptr, *w = nullptr;
     double* ph = nullptr;
     int rc = dalloc(c, &f, spec_elems(c));
@@ -583,10 +655,10 @@ int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride
     if (!rc) rc = upload_spec(c, f, spec, stride);
     if (!rc) {
         dim3 g((d.ny + TB - 1) / TB, d.hnx, 1);
-        if (padded) k_cheb_s2p<1><<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(f, 0, y, 0, d, plz, TB, ld);
-        else k_cheb_s2p<0><<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(f, 0, y, 0, d, plz, TB, ld);
+        if (padded) k_cheb_s2p<1><<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(f, 0, y, 0, d, plz, tw4, TB, ld);
+        else k_cheb_s2p<0><<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(f, 0, y, 0, d, plz, tw4, TB, ld);
         dim3 g2((d.nxh + c->TK_inv - 1) / c->TK_inv, planes, 1);
-        k_span_inv<<<g2, kFftThreads, sizeof(double2) * c->TK_inv * c->ld_my, c->stream>>>(y, 0, w, 0, d, planes,
+        k_span_inv<0><<<g2, kFftThreads, sizeof(double2) * c->TK_inv * c->ld_my, c->stream>>>(y, 0, w, 0, d, planes,
                                                                                            c->pl_y, c->TK_inv, c->ld_my);
         dim3 g3((d.my / 2 + RP1 - 1) / RP1, planes);
         k_x_c2r<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(w, ph, d, c->pl_x, RP1, c->ld_mx);
@@ -624,7 +696,7 @@ int rpcf_phys_to_spec(rpcf_ctx* c, const double* phys, double* spec, long stride
         k_x_r2c<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(ph, z, c->fac, d, c->pl_x, RP1,
                                                                                   c->ld_mx);
         dim3 g2((d.hnx + c->TK_fwd - 1) / c->TK_fwd, planes, 1);
-        k_span_fwd<<<g2, kFftThreads, sizeof(double2) * c->TK_fwd * c->ld_my, c->stream>>>(z, 0, y, 0, d, planes,
+        k_span_fwd<0><<<g2, kFftThreads, sizeof(double2) * c->TK_fwd * c->ld_my, c->stream>>>(z, 0, y, 0, d, planes,
                                                                                            c->pl_y, c->TK_fwd, c->ld_my);
         dim3 g((d.ny + TB - 1) / TB, d.hnx);
         k_cheb_p2s<<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(y, f, d, planes, plz, TB, ld);

@@ -88,6 +88,7 @@ def load(path):
         "rpcf_set_field": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_long]),
         "rpcf_get_field": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_long]),
         "rpcf_get_nonlinear": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_long]),
+        "rpcf_set_nonlinear": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_long]),
         "rpcf_nonlinear": (ctypes.c_int, [_vp]),
         "rpcf_linear": (ctypes.c_int, [_vp]),
         "rpcf_velocity": (ctypes.c_int, [_vp]),
@@ -193,6 +194,12 @@ class Simulation:
         self._check(self.dll.rpcf_get_nonlinear(self.h, int(prev), _ptr(v.view(np.float64)),
                                                 _ptr(e.view(np.float64)), self.s["nz"]), "get_nonlinear")
         return v, e
+
+    def set_nonlinear(self, n_v, n_eta, prev=False):
+        n_v = np.ascontiguousarray(n_v, np.complex128)
+        n_eta = np.ascontiguousarray(n_eta, np.complex128)
+        self._check(self.dll.rpcf_set_nonlinear(self.h, int(prev), _ptr(n_v.view(np.float64)),
+                                                _ptr(n_eta.view(np.float64)), n_v.shape[1]), "set_nonlinear")
 
     # -- stepping -----------------------------------------------------------
     def nonlinear(self):

@@ -105,6 +105,9 @@ int rpcf_get_field(rpcf_ctx* ctx, int field, double* f_hat, long pencil_stride);
  * (host nonlinear_v / nonlinear_omega_y, source/rhs.cpp:107-117); prev=1 reads the
  * previous step's (nonlinear_*_p). */
 int rpcf_get_nonlinear(rpcf_ctx* ctx, int prev, double* n_v, double* n_eta, long pencil_stride);
+/* Restore them (recovery: read_recover_data, source/output.cpp:141-178, which reloads
+ * nonlinear_v / nonlinear_omega_y so that the next step's swap makes them the history). */
+int rpcf_set_nonlinear(rpcf_ctx* ctx, int prev, const double* n_v, const double* n_eta, long pencil_stride);
 
 /* = getNonlinear + unpadded Chebyshev s2p of N_v, N_eta + AB2 history swap + (0,0)
  * Lamb save (source/nonlinear.cu:25-108; source/rhs.cpp:83-127, 229-236). */

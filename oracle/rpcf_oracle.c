@@ -1066,6 +1066,17 @@ int rpcf_get_nonlinear(rpcf_ctx* c, int prev, double* n_v, double* n_eta, long s
     return RPCF_OK;
 }
 
+/* read_recover_data (source/output.cpp:141-178): nonlinear_* and the (0,0) Lamb copies */
+int rpcf_set_nonlinear(rpcf_ctx* c, int prev, const double* n_v, const double* n_eta, long stride) {
+    if (!c) return RPCF_EINVAL;
+    if (copy_in(c, prev ? c->nl_vp : c->nl_v, n_v, stride) || copy_in(c, prev ? c->nl_op : c->nl_o, n_eta, stride))
+        return set_err(c, RPCF_EINVAL, "rpcf_set_nonlinear: bad array");
+    cx* lx = prev ? c->lambx0_p : c->lambx0;
+    cx* lz = prev ? c->lambz0_p : c->lambz0;
+    for (int i = 0; i < c->nz; i++) { lx[i] = (prev ? c->nl_vp : c->nl_v)[i]; lz[i] = (prev ? c->nl_op : c->nl_o)[i]; }
+    return RPCF_OK;
+}
+
 int rpcf_phys_to_spec(rpcf_ctx* c, const double* phys, double* spec, long stride, int padded) {
     if (!c) return RPCF_EINVAL;
     if (!phys || !spec || stride < c->nz) return set_err(c, RPCF_EINVAL, "rpcf_phys_to_spec: bad array");
