@@ -62,7 +62,8 @@ def kernel_bytes(s, prm):
         "k_lamb_phys": 6 * Wb + 3 * Zb,
         "k_span_fwd": 3 * Zb + 3 * Sp,
         "k_p2s_nl": 3 * Sp + 2 * S,
-        "k_linear": 2 * nslot * M1 + 8 * S,
+        "k_lin_tv": 2 * S + 5 * S,
+        "k_lin_solve": 2 * nslot * M1 + 11 * S,
         "k_velocity": 8 * S,
     }
 

@@ -260,240 +260,235 @@ struct LinState {
     int flow_control;
 };
 
-// Batched GEMV of the two pencils of a slot: y_q = M x_q, M column-major n x n.
-// Wave w of the block sums the columns j in its quarter of the range for the RPL rows
-// lane, lane+64, ... (coalesced 1 KB loads, RPL*J/4 of them independent per lane); the
-// four partial sums are added through LDS (part[4][2][nz]) in wave order.
-template <int RPL>
-__device__ __forceinline__ void gemv_slot(const double2* __restrict__ M, const double2* x0, const double2* x1, int n,
-                                          double2* part, double2* y0, double2* y1) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int jb = (n * w) / 4, je = (n * (w + 1)) / 4;
-    double2 a0[RPL], a1[RPL];
+// -------------------------------------------------------------------------------------
+// Operator products of the Crank-Nicolson right-hand sides for every pencil:
+//   W[0] = T0 v, W[1] = T2 v, W[2] = T4q v, W[3] = T0 eta, W[4] = T2q eta   (W[m][P][nz])
+// with T4q = T4*dt*0.5/Re and T2q = T2*0.5/Re*dt (the quotients of source/linear.cpp:100,
+// 142).  The reference rebuilds B_rj = f(T0, T2, T4) per element (source/linear.cpp:95-107,
+// 137-153); by linearity sum_j v_j B_rj is a combination of these row products, so one
+// pass over the operator tables serves the NPB pencils of a block (real-matrix times
+// complex-vector products, fused multiply-adds).  Register tile: a thread owns 4 rows x
+// 8 pencils of one product m, so each vector element read from LDS feeds 8 FMAs
+// (tables transposed: Tt[j*nz + p], vectors in LDS as [j][pencil]).
+// -------------------------------------------------------------------------------------
+constexpr int kTvRows = 4, kTvCols = 8;
+
+template <int NPB>
+__global__ __launch_bounds__(384) void k_lin_tv(const double2* __restrict__ sv, const double2* __restrict__ se,
+                                                double2* __restrict__ W, Dims d, const double* __restrict__ T0t,
+                                                const double* __restrict__ T2t, const double* __restrict__ T4q,
+                                                const double* __restrict__ T2q) {
+    static_assert(NPB % kTvCols == 0, "pencil tile");
+    extern __shared__ __align__(16) unsigned char tv_lds_raw[];
+    constexpr int LDX = NPB + 1;  // padded row: the transposing stores hit different banks
+    double2* xv = (double2*)tv_lds_raw;  // [nz][LDX]
+    double2* xe = xv + (size_t)d.nz * LDX;
+    const int n = d.nz, p0 = blockIdx.x * NPB;
+    const int np = min(NPB, d.P - p0);
+    for (int i = threadIdx.x; i < NPB * n; i += blockDim.x) {
+        const int q = i / n, j = i - q * n;
+        const bool ok = q < np;
+        xv[j * LDX + q] = ok ? sv[(long)(p0 + q) * n + j] : make_double2(0.0, 0.0);
+        xe[j * LDX + q] = ok ? se[(long)(p0 + q) * n + j] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    constexpr int NH = NPB / kTvCols;
+    const int ngr = (n + kTvRows - 1) / kTvRows;
+    for (int u = threadIdx.x; u < 5 * ngr * NH; u += blockDim.x) {
+        const int m = u / (ngr * NH), rem = u - m * ngr * NH;
+        const int g = rem / NH, h = rem - g * NH;
+        const int pb = g * kTvRows, qb = h * kTvCols;
+        const double* T = m == 0 || m == 3 ? T0t : (m == 1 ? T2t : (m == 2 ? T4q : T2q));
+        const double2* X = (m < 3 ? xv : xe) + qb;
+        double ar[kTvRows][kTvCols], ai[kTvRows][kTvCols];
 #pragma unroll
-    for (int i = 0; i < RPL; i++) a0[i] = a1[i] = make_double2(0.0, 0.0);
-#pragma unroll 2
-    for (int j = jb; j < je; j++) {
-        const double2 u0 = x0[j], u1 = x1[j];
-        const double2* col = M + (long)j * n;
+        for (int i = 0; i < kTvRows; i++)
 #pragma unroll
-        for (int i = 0; i < RPL; i++) {
-            const int r = lane + 64 * i;
-            if (r < n) {
-                const double2 m = col[r];
-                a0[i].x = fma(m.x, u0.x, fma(-m.y, u0.y, a0[i].x));
-                a0[i].y = fma(m.x, u0.y, fma(m.y, u0.x, a0[i].y));
-                a1[i].x = fma(m.x, u1.x, fma(-m.y, u1.y, a1[i].x));
-                a1[i].y = fma(m.x, u1.y, fma(m.y, u1.x, a1[i].y));
+            for (int q = 0; q < kTvCols; q++) ar[i][q] = ai[i][q] = 0.0;
+        // the table column of j+1 is loaded while column j is used
+        double cn[kTvRows];
+#pragma unroll
+        for (int i = 0; i < kTvRows; i++) cn[i] = pb + i < n ? T[pb + i] : 0.0;
+        for (int j = 0; j < n; j++) {
+            double c[kTvRows];
+#pragma unroll
+            for (int i = 0; i < kTvRows; i++) {
+                c[i] = cn[i];
+                cn[i] = (j + 1 < n && pb + i < n) ? T[(long)(j + 1) * n + pb + i] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < kTvCols; q++) {
+                const double2 x = X[j * LDX + q];
+#pragma unroll
+                for (int i = 0; i < kTvRows; i++) {
+                    ar[i][q] = fma(c[i], x.x, ar[i][q]);
+                    ai[i][q] = fma(c[i], x.y, ai[i][q]);
+                }
             }
         }
-    }
+        double2* out = W + (long)m * d.P * n;
 #pragma unroll
-    for (int i = 0; i < RPL; i++) {
-        const int r = lane + 64 * i;
-        if (r < n) {
-            part[(w * 2 + 0) * n + r] = a0[i];
-            part[(w * 2 + 1) * n + r] = a1[i];
-        }
+        for (int q = 0; q < kTvCols; q++)
+            if (qb + q < np)
+#pragma unroll
+                for (int i = 0; i < kTvRows; i++)
+                    if (pb + i < n) out[(long)(p0 + qb + q) * n + pb + i] = make_double2(ar[i][q], ai[i][q]);
     }
-    __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        double2 s0 = part[r], s1 = part[n + r];
-        for (int q = 1; q < 4; q++) {
-            s0 = zadd(s0, part[(q * 2) * n + r]);
-            s1 = zadd(s1, part[(q * 2 + 1) * n + r]);
-        }
-        y0[r] = s0;
-        y1[r] = s1;
-    }
-    __syncthreads();
 }
 
-// Mean-flow equations of the (0,0) slot: _get_linear_u0/_w0 (source/linear.cpp:173-205),
-// their solves, and the constant-flow-rate correction of u0 (u0 += f u_h with u_h the
-// response to a unit uniform forcing and f restoring the initial bulk).  Runs as block 0
-// of k_linear so that it overlaps the per-mode blocks.
-template <int RPL>
-__device__ void linear_mean_block(const LinState& st, const Dims& d, const LinParams& lp, const double* __restrict__ T0t,
-                                  const double* __restrict__ T2t, const double* __restrict__ qw, double2* lds) {
-    const int n = d.nz, N = d.N, r = threadIdx.x;
-    const double Re = lp.Re, dt = lp.dt;
-    const long nn = (long)n * n;
-    double2* vn = lds;
-    double2* rhs0 = lds + n;
-    double2* rhs1 = lds + 2 * n;
-    double2* x0 = lds + 3 * n;
-    double2* x1 = lds + 4 * n;
-    double2* sf = lds + 5 * n;
-    double2* part = lds + 8 * n;
-    for (int e = 0; e < 2; e++) {
-        double2* s = e ? st.se : st.sv;
-        const double2* l0 = e ? st.nle : st.nlv;
-        const double2* l0p = e ? st.nlep : st.nlvp;
-        if (r < n) vn[r] = s[r];
-        __syncthreads();
-        if (r < n) {
-            double2 v = make_double2(0.0, 0.0);
-            if (r >= 2) {
-                double2 acc = make_double2(0.0, 0.0);
+// y_q[r] = sum_j M[r][j] x_q[j] for the two vectors of a slot, thread r = row r, j
+// ascending; M column-major, so a wave reads a contiguous column segment per j.
+__device__ __forceinline__ void gemv_rows2(const double2* __restrict__ M, const double2* x0, const double2* x1, int r,
+                                           int n, double2* y0, double2* y1) {
+    double a0r = 0, a0i = 0, a1r = 0, a1i = 0;
 #pragma unroll 8
-                for (int j = 0; j <= N; j++) {
-                    const long i1 = (long)n * j + (r - 1);
-                    acc = zadd(acc, zmul(vn[j], make_double2(T0t[i1] + dt * 0.5 / Re * T2t[i1], 0.0)));
-                }
-                v = zsub(acc, zmulr(zsub(zmulr(l0[r - 1], 1.5), zmulr(l0p[r - 1], 0.5)), dt));
-            }
-            rhs0[r] = v;
-            rhs1[r] = make_double2(r >= 2 ? dt : 0.0, 0.0);  // unit uniform forcing
-        }
-        __syncthreads();
-        gemv_slot<RPL>(st.minv + (e ? (long)d.nslot * nn : 0), rhs0, rhs1, n, part, x0, x1);
-        if (e == 0 && st.flow_control == 1) {
-            if (r == 0) {
-                double2 qp = make_double2(0.0, 0.0);
-                double qh = 0.0;
-                for (int k = 0; k < n; k++) {
-                    qp = zadd(qp, zmulr(x0[k], qw[k]));
-                    qh = qh + x1[k].x * qw[k];
-                }
-                sf[0] = make_double2((st.qw_target_re - qp.x) / qh, (st.qw_target_im - qp.y) / qh);
-            }
-            __syncthreads();
-            if (r < n) x0[r] = zadd(x0[r], zmul(sf[0], x1[r]));
-        }
-        if (r < n) s[r] = x0[r];
-        __syncthreads();
+    for (int j = 0; j < n; j++) {
+        const double2 m = M[(long)j * n + r];
+        const double2 u0 = x0[j], u1 = x1[j];
+        a0r = fma(m.x, u0.x, fma(-m.y, u0.y, a0r));
+        a0i = fma(m.x, u0.y, fma(m.y, u0.x, a0i));
+        a1r = fma(m.x, u1.x, fma(-m.y, u1.y, a1r));
+        a1i = fma(m.x, u1.y, fma(m.y, u1.x, a1i));
     }
+    *y0 = make_double2(a0r, a0i);
+    *y1 = make_double2(a1r, a1i);
 }
 
 // -------------------------------------------------------------------------------------
-// One Crank-Nicolson/AB2 update of both prognostic equations for the one or two pencils
-// sharing a matrix slot (every slot but (0,0)): _get_linear_v + AB2
-// (source/linear.cpp:86-125, source/rhs.cpp:205-215), v solve (source/solver.cpp:107-123),
-// _get_linear_omega_y (source/linear.cpp:127-171), eta solve.
-// The right-hand sides use the linearity of the reference's per-element operator,
-//   sum_j v_j B_rj = (T4q v)_p + c2 (T2 v)_p - c0 (T0 v)_p + i [a2_p (T2 v)_p + a0_p (T0 v)_p],
-// so each row needs three real dot products per pencil (fused multiply-adds) instead of
-// rebuilding B_rj; T0, T2 and the quotients T4q = T4*dt*0.5/Re, T2q = T2*0.5/Re*dt arrive
-// transposed (element (p, j) at j*nz + p: consecutive rows, consecutive addresses).
-// Block of 256 threads; rows r < nz map to threads.
+// Crank-Nicolson/AB2 update of the one or two pencils of a matrix slot from the operator
+// products W (k_lin_tv): v right-hand side (_get_linear_v + AB2, source/linear.cpp:86-125,
+// source/rhs.cpp:205-215), v solve (solveEq, source/solver.cpp:107-123), T0 v^{n+1} (one
+// pass over T0), eta right-hand side (_get_linear_omega_y, source/linear.cpp:127-171), eta
+// solve.  Block 0 runs the mean flow of slot (0,0) (_get_linear_u0/_w0,
+// source/linear.cpp:173-205) and the constant-flow-rate correction.  Thread r = row r.
+//   v rows r = 4..N, p = r-2:  (T4q v + k2 T2 v - k0 T0 v) + i (a2 T2 v + a0 T0 v)
+//   eta rows r = 2..N, p = r-1: (T2q eta + c0 T0 eta) + i ci T0 eta + i e T0 (v^n + v^{n+1})
 // -------------------------------------------------------------------------------------
-// LDS (dynamic, 16*nz*16 bytes): vn, en, rhs, vnew (two pencils each) and part[4][2][nz].
-// Block 0 runs the mean-flow equations of slot (0,0); block b > 0 runs slot b.
-template <int RPL>
-__global__ __launch_bounds__(256, 4) void k_linear(LinState st, Dims d, LinParams lp, const double* __restrict__ T0,
-                                                   const double* __restrict__ T2, const double* __restrict__ T4q,
-                                                   const double* __restrict__ T2q, const double* __restrict__ U,
+__global__ __launch_bounds__(256) void k_lin_solve(LinState st, const double2* __restrict__ W, Dims d, LinParams lp,
+                                                   const double* __restrict__ T0t, const double* __restrict__ U,
                                                    const double* __restrict__ dU, const double* __restrict__ ddU,
                                                    const double* __restrict__ qw) {
-    extern __shared__ __align__(16) unsigned char lin_lds_raw[];
-    const int n = d.nz, N = d.N, r = threadIdx.x;
-    double2* lds = (double2*)lin_lds_raw;
-    double2* vn[2] = {lds, lds + n};
-    double2* en[2] = {lds + 2 * n, lds + 3 * n};
-    double2* rhs[2] = {lds + 4 * n, lds + 5 * n};
-    double2* vnew[2] = {lds + 6 * n, lds + 7 * n};
-    double2* part = lds + 8 * n;
-    if (blockIdx.x == 0) {
-        linear_mean_block<RPL>(st, d, lp, T0, T2, qw, lds);
-        return;
-    }
+    __shared__ double2 rhs[2][256], vnew[2][256];
+    __shared__ double2 s_f;
     const int slot = blockIdx.x;
     const int kyh = slot / d.hnx, kx = slot - kyh * d.hnx;
+    const int n = d.nz, r = threadIdx.x;
     const double Re = lp.Re, dt = lp.dt;
-    const long nn = (long)n * n;
+    const long nn = (long)n * n, PS = (long)d.P * n;
     const double2* Mv = st.minv + (long)slot * nn;
     const double2* Mo = st.minv + ((long)d.nslot + slot) * nn;
-    // pencils ky = kyh and ky = ny - kyh share the slot; with kyh = 0 or ny/2 the slot has
-    // one pencil and the second lane repeats it without being written back
+    const double2 *W0 = W, *W1 = W + PS, *W2 = W + 2 * PS, *W3 = W + 3 * PS, *W4 = W + 4 * PS;
+
+    if (slot == 0) {
+        // u0: rhs_r = (T0 u0 + dt/2/Re T2 u0)_{r-1} - dt (1.5 Lx0 - 0.5 Lx0_prev)_{r-1}, rows 0,1 = 0;
+        // w0 likewise with eta's products and Ly0.  Second vector: unit uniform forcing.
+        const double h = dt * 0.5 / Re;
+        for (int e = 0; e < 2; e++) {
+            double2* s = e ? st.se : st.sv;
+            const double2* l0 = e ? st.nle : st.nlv;
+            const double2* l0p = e ? st.nlep : st.nlvp;
+            if (r < n) {
+                double2 v = make_double2(0.0, 0.0);
+                if (r >= 2) {
+                    const double2 acc = e ? zadd(W3[r - 1], W4[r - 1]) : zadd(W0[r - 1], zmulr(W1[r - 1], h));
+                    v = zsub(acc, zmulr(zsub(zmulr(l0[r - 1], 1.5), zmulr(l0p[r - 1], 0.5)), dt));
+                }
+                rhs[0][r] = v;
+                rhs[1][r] = make_double2(r >= 2 ? dt : 0.0, 0.0);
+            }
+            __syncthreads();
+            double2 x = make_double2(0.0, 0.0), uh = make_double2(0.0, 0.0);
+            if (r < n) {
+                gemv_rows2(e ? Mo : Mv, rhs[0], rhs[1], r, n, &x, &uh);
+                vnew[0][r] = x;
+                vnew[1][r] = uh;
+            }
+            __syncthreads();
+            if (e == 0 && st.flow_control == 1) {
+                if (r == 0) {
+                    double2 qp = make_double2(0.0, 0.0);
+                    double qh = 0.0;
+                    for (int k = 0; k < n; k++) {
+                        qp = zadd(qp, zmulr(vnew[0][k], qw[k]));
+                        qh = qh + vnew[1][k].x * qw[k];
+                    }
+                    s_f = make_double2((st.qw_target_re - qp.x) / qh, (st.qw_target_im - qp.y) / qh);
+                }
+                __syncthreads();
+                if (r < n) x = zadd(x, zmul(s_f, uh));
+            }
+            if (r < n) s[r] = x;
+            __syncthreads();
+        }
+        return;
+    }
+
     const bool two = !(kyh == 0 || kyh == d.ny / 2);
-    const long po0 = ((long)kyh * d.hnx + kx) * n;
-    const long po1 = two ? ((long)(d.ny - kyh) * d.hnx + kx) * n : po0;
+    const long c0 = (long)kyh * d.hnx + kx;
+    const long c1 = two ? (long)(d.ny - kyh) * d.hnx + kx : c0;
+    const long po0 = c0 * n, po1 = c1 * n;
     double alpha, beta0, beta1, a_unused;
     ab_of(d, lp, kx, kyh, &alpha, &beta0);
     ab_of(d, lp, kx, two ? d.ny - kyh : kyh, &a_unused, &beta1);
     const double kmn = alpha * alpha + beta0 * beta0;
     if (r < n) {
-        vn[0][r] = st.sv[po0 + r];
-        en[0][r] = st.se[po0 + r];
-        vn[1][r] = st.sv[po1 + r];
-        en[1][r] = st.se[po1 + r];
-    }
-    __syncthreads();
-    // v right-hand side, rows 4..N at collocation point p = r-2; rows 0..3 (BCs) zero
-    if (r < n) {
         double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
         if (r >= 4) {
-            const int pr = r - 2;
-            double s0r = 0, s0i = 0, s2r = 0, s2i = 0, s4r = 0, s4i = 0;  // pencil 0: T0 v, T2 v, T4q v
-            double t0r = 0, t0i = 0, t2r = 0, t2i = 0, t4r = 0, t4i = 0;  // pencil 1
-#pragma unroll 2
-            for (int j = 0; j <= N; j++) {
-                const long ij = (long)n * j + pr;
-                const double c0 = T0[ij], c2 = T2[ij], c4 = T4q[ij];
-                const double2 x = vn[0][j], y = vn[1][j];
-                s0r = fma(c0, x.x, s0r); s0i = fma(c0, x.y, s0i);
-                s2r = fma(c2, x.x, s2r); s2i = fma(c2, x.y, s2i);
-                s4r = fma(c4, x.x, s4r); s4i = fma(c4, x.y, s4i);
-                t0r = fma(c0, y.x, t0r); t0i = fma(c0, y.y, t0i);
-                t2r = fma(c2, y.x, t2r); t2i = fma(c2, y.y, t2i);
-                t4r = fma(c4, y.x, t4r); t4i = fma(c4, y.y, t4i);
-            }
+            const int p = r - 2;
             const double k2 = (1 - kmn * dt / Re), k0 = kmn * (1 - kmn * dt * 0.5 / Re);
-            const double i2 = -alpha * dt * 0.5 * U[pr];
-            const double i0 = (kmn * alpha * dt * 0.5 * U[pr] + alpha * dt * 0.5 * ddU[pr]);
-            // (B v) = (T4q v + k2 T2 v - k0 T0 v) + i (i2 T2 v + i0 T0 v), with complex v
-            const double br0 = s4r + k2 * s2r - k0 * s0r, bi0 = s4i + k2 * s2i - k0 * s0i;
-            const double br1 = t4r + k2 * t2r - k0 * t0r, bi1 = t4i + k2 * t2i - k0 * t0i;
-            a0 = make_double2(br0 - (i2 * s2i + i0 * s0i), bi0 + (i2 * s2r + i0 * s0r));
-            a1 = make_double2(br1 - (i2 * t2i + i0 * t0i), bi1 + (i2 * t2r + i0 * t0r));
-            a0 = zadd(a0, zmulr(zsub(zmulr(st.nlv[po0 + pr], 1.5), zmulr(st.nlvp[po0 + pr], 0.5)), dt));
-            a1 = zadd(a1, zmulr(zsub(zmulr(st.nlv[po1 + pr], 1.5), zmulr(st.nlvp[po1 + pr], 0.5)), dt));
+            const double i2 = -alpha * dt * 0.5 * U[p];
+            const double i0 = (kmn * alpha * dt * 0.5 * U[p] + alpha * dt * 0.5 * ddU[p]);
+            const double2 s0 = W0[po0 + p], s2 = W1[po0 + p], s4 = W2[po0 + p];
+            const double2 t0 = W0[po1 + p], t2 = W1[po1 + p], t4 = W2[po1 + p];
+            a0 = make_double2(s4.x + k2 * s2.x - k0 * s0.x - (i2 * s2.y + i0 * s0.y),
+                              s4.y + k2 * s2.y - k0 * s0.y + (i2 * s2.x + i0 * s0.x));
+            a1 = make_double2(t4.x + k2 * t2.x - k0 * t0.x - (i2 * t2.y + i0 * t0.y),
+                              t4.y + k2 * t2.y - k0 * t0.y + (i2 * t2.x + i0 * t0.x));
+            a0 = zadd(a0, zmulr(zsub(zmulr(st.nlv[po0 + p], 1.5), zmulr(st.nlvp[po0 + p], 0.5)), dt));
+            a1 = zadd(a1, zmulr(zsub(zmulr(st.nlv[po1 + p], 1.5), zmulr(st.nlvp[po1 + p], 0.5)), dt));
         }
         rhs[0][r] = a0;
         rhs[1][r] = a1;
     }
     __syncthreads();
-    gemv_slot<RPL>(Mv, rhs[0], rhs[1], n, part, vnew[0], vnew[1]);
-    // eta right-hand side, rows 2..N at p = r-1; rows 0,1 zero:
-    //   sum_j eta_j C_rj + (v_j + vnew_j) E_rj with C = T2q + c0 T0 + i ci T0, E = i e T0
+    if (r < n) gemv_rows2(Mv, rhs[0], rhs[1], r, n, &vnew[0][r], &vnew[1][r]);
+    __syncthreads();
     if (r < n) {
         double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
         if (r >= 2) {
-            const int pr = r - 1;
-            double s0r = 0, s0i = 0, s2r = 0, s2i = 0, w0r = 0, w0i = 0;  // pencil 0: T0 eta, T2q eta, T0 (v+vnew)
-            double t0r = 0, t0i = 0, t2r = 0, t2i = 0, z0r = 0, z0i = 0;  // pencil 1
-#pragma unroll 2
-            for (int j = 0; j <= N; j++) {
-                const long ij = (long)n * j + pr;
-                const double c0 = T0[ij], c2 = T2q[ij];
-                const double2 x = en[0][j], y = en[1][j];
-                const double2 vx = zadd(vn[0][j], vnew[0][j]), vy = zadd(vn[1][j], vnew[1][j]);
-                s0r = fma(c0, x.x, s0r); s0i = fma(c0, x.y, s0i);
-                s2r = fma(c2, x.x, s2r); s2i = fma(c2, x.y, s2i);
-                w0r = fma(c0, vx.x, w0r); w0i = fma(c0, vx.y, w0i);
-                t0r = fma(c0, y.x, t0r); t0i = fma(c0, y.y, t0i);
-                t2r = fma(c2, y.x, t2r); t2i = fma(c2, y.y, t2i);
-                z0r = fma(c0, vy.x, z0r); z0i = fma(c0, vy.y, z0i);
+            const int p = r - 1;
+            // T0 v^{n+1} at row p
+            double w0r = 0, w0i = 0, w1r = 0, w1i = 0;
+#pragma unroll 4
+            for (int j = 0; j < n; j++) {
+                const double c = T0t[(long)j * n + p];
+                const double2 x = vnew[0][j], y = vnew[1][j];
+                w0r = fma(c, x.x, w0r);
+                w0i = fma(c, x.y, w0i);
+                w1r = fma(c, y.x, w1r);
+                w1i = fma(c, y.y, w1i);
             }
-            const double c0k = (1 - kmn * dt * 0.5 / Re), ci = -alpha * dt * 0.5 * U[pr];
-            const double e0 = -0.5 * beta0 * dU[pr] * dt, e1 = -0.5 * beta1 * dU[pr] * dt;
-            // C eta = (T2q eta + c0k T0 eta) + i ci T0 eta ; E w = i e T0 w
-            a0 = make_double2(s2r + c0k * s0r - ci * s0i - e0 * w0i, s2i + c0k * s0i + ci * s0r + e0 * w0r);
-            a1 = make_double2(t2r + c0k * t0r - ci * t0i - e1 * z0i, t2i + c0k * t0i + ci * t0r + e1 * z0r);
-            a0 = zadd(a0, zmulr(zsub(zmulr(st.nle[po0 + pr], 1.5), zmulr(st.nlep[po0 + pr], 0.5)), dt));
-            a1 = zadd(a1, zmulr(zsub(zmulr(st.nle[po1 + pr], 1.5), zmulr(st.nlep[po1 + pr], 0.5)), dt));
+            const double c0k = (1 - kmn * dt * 0.5 / Re), ci = -alpha * dt * 0.5 * U[p];
+            const double e0 = -0.5 * beta0 * dU[p] * dt, e1 = -0.5 * beta1 * dU[p] * dt;
+            const double2 s0 = W3[po0 + p], s2 = W4[po0 + p], sv0 = W0[po0 + p];
+            const double2 t0 = W3[po1 + p], t2 = W4[po1 + p], sv1 = W0[po1 + p];
+            const double2 u0 = make_double2(sv0.x + w0r, sv0.y + w0i), u1 = make_double2(sv1.x + w1r, sv1.y + w1i);
+            a0 = make_double2(s2.x + c0k * s0.x - ci * s0.y - e0 * u0.y, s2.y + c0k * s0.y + ci * s0.x + e0 * u0.x);
+            a1 = make_double2(t2.x + c0k * t0.x - ci * t0.y - e1 * u1.y, t2.y + c0k * t0.y + ci * t0.x + e1 * u1.x);
+            a0 = zadd(a0, zmulr(zsub(zmulr(st.nle[po0 + p], 1.5), zmulr(st.nlep[po0 + p], 0.5)), dt));
+            a1 = zadd(a1, zmulr(zsub(zmulr(st.nle[po1 + p], 1.5), zmulr(st.nlep[po1 + p], 0.5)), dt));
         }
         rhs[0][r] = a0;
         rhs[1][r] = a1;
     }
     __syncthreads();
-    gemv_slot<RPL>(Mo, rhs[0], rhs[1], n, part, en[0], en[1]);
     if (r < n) {
-        st.se[po0 + r] = en[0][r];
+        double2 e0, e1;
+        gemv_rows2(Mo, rhs[0], rhs[1], r, n, &e0, &e1);
+        st.se[po0 + r] = e0;
         st.sv[po0 + r] = vnew[0][r];
         if (two) {
-            st.se[po1 + r] = en[1][r];
+            st.se[po1 + r] = e1;
             st.sv[po1 + r] = vnew[1][r];
         }
     }

@@ -18,9 +18,11 @@
 
 namespace {
 
-enum KernelId { KID_VELOCITY = 0, KID_S2P, KID_SPAN_INV, KID_LAMB, KID_SPAN_FWD, KID_P2S_NL, KID_LINEAR, KID_COUNT };
+enum KernelId {
+    KID_VELOCITY = 0, KID_S2P, KID_SPAN_INV, KID_LAMB, KID_SPAN_FWD, KID_P2S_NL, KID_LIN_TV, KID_LINEAR, KID_COUNT
+};
 const char* const kKernelNames[KID_COUNT] = {"k_velocity", "k_cheb_s2p", "k_span_inv", "k_lamb_phys",
-                                             "k_span_fwd", "k_p2s_nl",   "k_linear"};
+                                             "k_span_fwd", "k_p2s_nl",   "k_lin_tv",   "k_lin_solve"};
 
 constexpr int kFftThreads = 256;
 
@@ -53,6 +55,7 @@ struct rpcf_ctx {
     double2* W = nullptr;   // [6][pz][my][nxh]
     double2* Z2 = nullptr;  // [3][pz][my][hnx]
     double2* minv = nullptr;
+    double2* Wlin = nullptr;  // [5][P][nz] operator products of the linear right-hand sides
     double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *U = nullptr, *dU = nullptr, *ddU = nullptr;
     // transposed tables for k_linear: T0, T2, and the quotients T4*dt*0.5/Re, T2*0.5/Re*dt
     double *T0t = nullptr, *T2t = nullptr, *T4q = nullptr, *T2q = nullptr;
@@ -305,6 +308,8 @@ int enq_nonlinear(rpcf_ctx* c) {
     return rc;
 }
 
+constexpr int kTvPencils = 16;  // pencils per block of k_lin_tv
+
 int enq_linear(rpcf_ctx* c) {
     const Dims d = c->d;
     LinState st;
@@ -312,24 +317,23 @@ int enq_linear(rpcf_ctx* c) {
     st.sv = c->sv; st.se = c->se; st.minv = c->minv;
     st.qw_target_re = c->q_target.x; st.qw_target_im = c->q_target.y;
     st.flow_control = c->prm.flow_control;
+    int rc = launch(c, KID_LIN_TV, [&] {
+        const int g = (d.P + kTvPencils - 1) / kTvPencils;
+        const int thr = 384;
+        const size_t lds = sizeof(double2) * 2 * (size_t)d.nz * (kTvPencils + 1);
+        k_lin_tv<kTvPencils><<<g, thr, lds, c->stream>>>(c->sv, c->se, c->Wlin, d, c->T0t, c->T2t, c->T4q, c->T2q);
+    });
+    if (rc) return rc;
     return launch(c, KID_LINEAR, [&] {
-        const int g = d.nslot;
-        const size_t lds = sizeof(double2) * 16 * (size_t)d.nz;
-#define RPCF_LIN(R) k_linear<R><<<g, 256, lds, c->stream>>>(st, d, c->lp, c->T0t, c->T2t, c->T4q, c->T2q, c->U, c->dU, c->ddU, c->qw)
-        switch ((d.nz + 63) / 64) {
-            case 1: RPCF_LIN(1); break;
-            case 2: RPCF_LIN(2); break;
-            case 3: RPCF_LIN(3); break;
-            default: RPCF_LIN(4); break;
-        }
-#undef RPCF_LIN
+        const int thr = 64 * ((d.nz + 63) / 64);
+        k_lin_solve<<<d.nslot, thr, 0, c->stream>>>(st, c->Wlin, d, c->lp, c->T0t, c->U, c->dU, c->ddU, c->qw);
     });
 }
 
 void free_ctx(rpcf_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->F, c->sv, c->se, c->nl[0], c->nl[1], c->nl[2], c->nl[3], c->Y, c->W, c->Z2, c->minv,
+    void* ptrs[] = {c->Wlin, c->F, c->sv, c->se, c->nl[0], c->nl[1], c->nl[2], c->nl[3], c->Y, c->W, c->Z2, c->minv,
                     c->T0, c->T2, c->T4, c->U, c->dU, c->ddU, c->meanU, c->qw, c->tw, c->T0t, c->T2t, c->T4q, c->T2q};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -401,6 +405,7 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     if ((rc = dalloc(c, &c->W, 6 * (size_t)d.pz * d.my * d.nxh))) return rc;
     if ((rc = dalloc(c, &c->Z2, 3 * (size_t)d.pz * d.my * d.hnx))) return rc;
     if ((rc = dalloc(c, &c->minv, 2 * (size_t)d.nslot * nz * nz))) return rc;
+    if ((rc = dalloc(c, &c->Wlin, 5 * P * nz))) return rc;
     if ((rc = dalloc(c, &c->T0, nz * nz)) || (rc = dalloc(c, &c->T2, nz * nz)) || (rc = dalloc(c, &c->T4, nz * nz)))
         return rc;
     if ((rc = dalloc(c, &c->T0t, nz * nz)) || (rc = dalloc(c, &c->T2t, nz * nz)) || (rc = dalloc(c, &c->T4q, nz * nz)) ||
@@ -492,6 +497,7 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     allow_lds(k_cheb_p2s, lim);
     allow_lds(k_x_c2r, lim);
     allow_lds(k_x_r2c, lim);
+    allow_lds(k_lin_tv<kTvPencils>, lim);
     (void)hipGetLastError();
     if (getenv("RPCF_VERBOSE"))
         fprintf(stderr, "rpcf: lds_limit=%d TB_s2p=%d TB_nl=%d TK=%d RP=%d nslot=%d\n", lim, c->TB_s2p, c->TB_nl,
