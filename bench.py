@@ -97,6 +97,20 @@ def cpu_baseline(prm, v0, e0, steps, ref_state):
     return threads, t_init, t_steps / steps, parity
 
 
+def copy_probe(nbytes, reps):
+    """a plain device-to-device copy of known size (torch's copy kernel): the calibration
+    run for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, which count what one dispatch of this
+    copy reads and writes against the nbytes it must move"""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +118,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one simulation on one GPU; multi-GPU runs are not "
               "part of it (the pencil/plane slab decomposition is design only). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe, 3)
+        return
 
     prm = rpcf.make_params(**C1)
     t0 = time.perf_counter()

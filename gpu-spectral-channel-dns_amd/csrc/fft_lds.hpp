@@ -214,11 +214,15 @@ constexpr int ct_rad(int n, int s) {
     return rad[s];
 }
 
-template <int R, int DIR, int N, int NS, typename C>
+// BT > 0: at most BT sequences and kFftBlock threads, so a thread owns at most
+// ceil(BT*N/R / kFftBlock) butterflies of a radix-R stage; BT = 0: RPCF_REGC/R.
+constexpr int kFftBlock = 256;
+
+template <int R, int DIR, int N, int NS, int BT, typename C>
 __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __restrict__ tw, int tid, int nthr) {
     constexpr int nR = N / R;
     constexpr int tstep = N / (NS * R);
-    constexpr int MAXPT = RPCF_REGC / R;
+    constexpr int MAXPT = BT ? (BT * nR + kFftBlock - 1) / kFftBlock : RPCF_REGC / R;
     const int tot = B * nR;
     C v[MAXPT][R];
 #pragma unroll
@@ -255,19 +259,20 @@ __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __r
     __syncthreads();
 }
 
-template <int N, int DIR, typename C, int S = 0, int NS = 1>
+template <int N, int DIR, int BT, typename C, int S = 0, int NS = 1>
 __device__ __forceinline__ void fft_ct(C* buf, int ld, int B, const C* tw, int tid, int nthr) {
     if constexpr (S < ct_nst(N)) {
         constexpr int R = ct_rad(N, S);
-        fft_stage_ct<R, DIR, N, NS>(buf, ld, B, tw, tid, nthr);
-        fft_ct<N, DIR, C, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
+        fft_stage_ct<R, DIR, N, NS, BT>(buf, ld, B, tw, tid, nthr);
+        fft_ct<N, DIR, BT, C, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
     }
 }
 
 // FN > 0: the compile-time plan of length FN (must equal pl.n); FN = 0: the runtime plan.
-template <int FN, int DIR, typename C>
+// BT > 0 (compiled plans only): B <= BT sequences and nthr == kFftBlock.
+template <int FN, int DIR, int BT = 0, typename C>
 __device__ __forceinline__ void fft_any(C* buf, int ld, int B, const FftPlan& pl, int tid, int nthr) {
-    if constexpr (FN > 0) fft_ct<FN, DIR>(buf, ld, B, (const C*)pl.tw, tid, nthr);
+    if constexpr (FN > 0) fft_ct<FN, DIR, BT>(buf, ld, B, (const C*)pl.tw, tid, nthr);
     else fft_run<DIR>(buf, ld, B, pl, tid, nthr);
 }
 

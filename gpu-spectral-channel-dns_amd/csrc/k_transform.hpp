@@ -19,6 +19,34 @@
 
 #define RPCF_LDS(T) extern __shared__ __align__(16) unsigned char rpcf_lds_raw[]; T* lds = (T*)rpcf_lds_raw
 
+// Pencils per block of the Chebyshev kernels: 8 x 16 B = one 128-byte line per point row
+// of the [kx][q][ky] layouts.  Columns per block of the span kernels.
+constexpr int kChebTile = 8;
+constexpr int kSpanTile = 16;
+constexpr int kLambPairs = 2;  // pairs of span rows per block of k_lamb_phys
+
+template <int V> struct IntC { static constexpr int value = V; };
+
+// Move `total` elements through registers with K loads in flight per thread before any is
+// consumed: a loop that loads, waits and stores one element at a time pays one memory
+// latency per element.
+template <int K, typename Load, typename Use>
+__device__ __forceinline__ void staged(int total, int tid, int nthr, Load load, Use use) {
+    for (int base = tid; base < total; base += K * nthr) {
+        double2 v[K];
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            const int i = base + u * nthr;
+            if (i < total) v[u] = load(i);
+        }
+#pragma unroll
+        for (int u = 0; u < K; u++) {
+            const int i = base + u * nthr;
+            if (i < total) use(i, v[u]);
+        }
+    }
+}
+
 // -------------------------------------------------------------------------------------
 // Chebyshev spectral -> physical along the wall-normal direction, f(z_q) = sum_k a_k T_k(z_q).
 // PAD=0: the nz points cos(pi p/N): cheby_pre_s2p_noPad + length-2N FFT of the even
@@ -61,26 +89,58 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
     RPCF_LDS(double2);
     const int f = blockIdx.z, kx = blockIdx.y, ky0 = blockIdx.x * TB;
     const int nb = min(TB, d.ny - ky0);
-    const int N = d.N, nz = d.nz, nout = PAD ? d.pz : d.nz;
+    const int N = FN ? FN / 2 : d.N, nz = N + 1, nout = PAD ? 2 * N + 1 : nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const long pstride = (long)d.hnx * nz;
     const double2* in = F + f * fstride + ((long)ky0 * d.hnx + kx) * nz;
     double2* out = Y + f * ystride + (long)kx * nout * d.ny + ky0;
+    const int total = nb * nz;
+    auto load = [&](int i) {
+        const int b = i / nz, k = i - b * nz;
+        return in[b * pstride + k];
+    };
+    auto ext = [&](int half, int i, double2 a) {
+        const int b = i / nz, k = i - b * nz;
+        if (half == 0) s2p_even_ext(lds + b * ld, a, k, N);
+        else s2p_odd_ext(lds + b * ld, a, k, N, tw4);
+    };
+    // with a compiled length and TB <= kChebTile the tile's coefficients stay in registers
+    // for both halves
+    constexpr int KC = FN ? (kChebTile * (FN / 2 + 1) + 255) / 256 : 1;
+    const bool inreg = FN > 0 && total <= KC * nthr;
+    double2 a[KC];
+    if (inreg) {
+#pragma unroll
+        for (int u = 0; u < KC; u++) {
+            const int i = tid + u * nthr;
+            if (i < total) a[u] = load(i);
+        }
+    }
+#pragma unroll
     for (int half = 0; half < (PAD ? 2 : 1); half++) {
-        for (int i = tid; i < nb * nz; i += nthr) {
-            const int b = i / nz, k = i - b * nz;
-            const double2 a = in[(long)b * d.hnx * nz + k];
-            if (half == 0) s2p_even_ext(lds + b * ld, a, k, N);
-            else s2p_odd_ext(lds + b * ld, a, k, N, tw4);
+        if (inreg) {
+#pragma unroll
+            for (int u = 0; u < KC; u++) {
+                const int i = tid + u * nthr;
+                if (i < total) ext(half, i, a[u]);
+            }
+        } else {
+            staged<8>(total, tid, nthr, load, [&](int i, double2 v) { ext(half, i, v); });
         }
         __syncthreads();
-        fft_any<FN, -1>(lds, ld, nb, pl, tid, nthr);
+        fft_any<FN, -1, kChebTile>(lds, ld, nb, pl, tid, nthr);
         // even half: points q = m (PAD=0) or q = 2m (PAD=1), m = 0..N; odd half: q = 2m+1, m < N
         const int cnt = half ? N : N + 1;
-        for (int i = tid; i < cnt * nb; i += nthr) {
-            const int m = i / nb, b = i - m * nb;
-            const int q = PAD ? 2 * m + half : m;
-            out[(long)q * d.ny + b] = lds[b * ld + m];
-        }
+        staged<8>(
+            cnt * nb, tid, nthr,
+            [&](int i) {
+                const int m = i / nb, b = i - m * nb;
+                return lds[b * ld + m];
+            },
+            [&](int i, double2 v) {
+                const int m = i / nb, b = i - m * nb;
+                out[(long)(PAD ? 2 * m + half : m) * d.ny + b] = v;
+            });
         __syncthreads();
     }
 }
@@ -97,20 +157,32 @@ __global__ __launch_bounds__(256) void k_span_inv(const double2* __restrict__ Y,
     RPCF_LDS(double2);
     const int f = blockIdx.z, q = blockIdx.y, kx0 = blockIdx.x * TK;
     const int nk = min(TK, d.nxh - kx0);
-    const int my = d.my, tid = threadIdx.x, nthr = blockDim.x;
+    const int my = FN ? FN : d.my, tid = threadIdx.x, nthr = blockDim.x;
     const double2* in = Y + f * ystride;
-    for (int i = tid; i < nk * my; i += nthr) {
-        const int kxi = i / my, s = i - kxi * my;
-        const int ky = ky_of_span(d, s);
-        lds[kxi * ld + s] = ky >= 0 ? in[((long)(kx0 + kxi) * planes + q) * d.ny + ky] : make_double2(0.0, 0.0);
-    }
+    staged<12>(
+        nk * my, tid, nthr,
+        [&](int i) {
+            const int kxi = i / my, s = i - kxi * my;
+            const int ky = ky_of_span(d, s);
+            return ky >= 0 ? in[((long)(kx0 + kxi) * planes + q) * d.ny + ky] : make_double2(0.0, 0.0);
+        },
+        [&](int i, double2 v) {
+            const int kxi = i / my, s = i - kxi * my;
+            lds[kxi * ld + s] = v;
+        });
     __syncthreads();
-    fft_any<FN, +1>(lds, ld, nk, pl, tid, nthr);
+    fft_any<FN, +1, kSpanTile>(lds, ld, nk, pl, tid, nthr);
     double2* out = W + f * wstride;
-    for (int i = tid; i < my * nk; i += nthr) {
-        const int s = i / nk, kxi = i - s * nk;
-        out[((long)q * my + s) * d.nxh + kx0 + kxi] = lds[kxi * ld + s];
-    }
+    staged<12>(
+        my * nk, tid, nthr,
+        [&](int i) {
+            const int s = i / nk, kxi = i - s * nk;
+            return lds[kxi * ld + s];
+        },
+        [&](int i, double2 v) {
+            const int s = i / nk, kxi = i - s * nk;
+            out[((long)q * my + s) * d.nxh + kx0 + kxi] = v;
+        });
 }
 
 // Load two half-spectrum rows (kx < nxh) as one packed length-mx spectrum
@@ -146,20 +218,26 @@ __device__ __forceinline__ void r2c_unpack(const double2* z, int k, int mx, doub
 // -------------------------------------------------------------------------------------
 template <int FN = 0>
 __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W, long wstride, double2* __restrict__ Z2, long zstride,
-                            const double* __restrict__ meanU, double Ro, double fac, Dims d, FftPlan pl, int RP,
-                            int ld) {
+                            int zld, const double* __restrict__ meanU, double Ro, double fac, Dims d, FftPlan pl,
+                            int RP, int ld) {
     RPCF_LDS(double2);
     const int q = blockIdx.y, rp0 = blockIdx.x * RP;
     const int nrp = min(RP, d.my / 2 - rp0);
-    const int mx = d.mx, my = d.my, nxh = d.nxh, tid = threadIdx.x, nthr = blockDim.x;
-    for (int i = tid; i < 6 * nrp * mx; i += nthr) {
-        const int seq = i / mx, k = i - seq * mx;
-        const int f = seq / nrp, ip = seq - f * nrp;
-        const double2* rA = W + f * wstride + ((long)q * my + 2 * (rp0 + ip)) * nxh;
-        lds[seq * ld + k] = c2r_pack(rA, rA + nxh, k, mx, nxh);
-    }
+    const int mx = FN ? FN : d.mx, my = d.my, nxh = d.nxh, tid = threadIdx.x, nthr = blockDim.x;
+    staged<9>(
+        6 * nrp * mx, tid, nthr,
+        [&](int i) {
+            const int seq = i / mx, k = i - seq * mx;
+            const int f = seq / nrp, ip = seq - f * nrp;
+            const double2* rA = W + f * wstride + ((long)q * my + 2 * (rp0 + ip)) * nxh;
+            return c2r_pack(rA, rA + nxh, k, mx, nxh);
+        },
+        [&](int i, double2 v) {
+            const int seq = i / mx, k = i - seq * mx;
+            lds[seq * ld + k] = v;
+        });
     __syncthreads();
-    fft_any<FN, +1>(lds, ld, 6 * nrp, pl, tid, nthr);
+    fft_any<FN, +1, 6 * kLambPairs>(lds, ld, 6 * nrp, pl, tid, nthr);
     const double mU = meanU[q];
     for (int i = tid; i < nrp * mx; i += nthr) {
         const int ip = i / mx, x = i - ip * mx;
@@ -174,16 +252,16 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W
         lds[(2 * nrp + ip) * ld + x] = make_double2(lza, lzb);
     }
     __syncthreads();
-    fft_any<FN, -1>(lds, ld, 3 * nrp, pl, tid, nthr);
+    fft_any<FN, -1, 3 * kLambPairs>(lds, ld, 3 * nrp, pl, tid, nthr);
     const int hnx = d.hnx;
     for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
         const int seq = i / hnx, k = i - seq * hnx;
         const int g = seq / nrp, ip = seq - g * nrp;
         double2 xa, xb;
         r2c_unpack(lds + seq * ld, k, mx, fac, &xa, &xb);
-        double2* o = Z2 + g * zstride + ((long)q * my + 2 * (rp0 + ip)) * hnx;
+        double2* o = Z2 + g * zstride + ((long)q * my + 2 * (rp0 + ip)) * zld;
         o[k] = xa;
-        o[hnx + k] = xb;
+        o[zld + k] = xb;
     }
 }
 
@@ -210,8 +288,8 @@ __global__ __launch_bounds__(256) void k_x_c2r(const double2* __restrict__ W, do
 }
 
 // x R2C of one field: phys[q][s][mx] -> Z2[q][s][hnx] with 1/(mx my) (rpcf_phys_to_spec).
-__global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, double2* __restrict__ Z2, double fac, Dims d, FftPlan pl,
-                        int RP, int ld) {
+__global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, double2* __restrict__ Z2, int zld, double fac, Dims d,
+                        FftPlan pl, int RP, int ld) {
     RPCF_LDS(double2);
     const int q = blockIdx.y, rp0 = blockIdx.x * RP;
     const int nrp = min(RP, d.my / 2 - rp0);
@@ -228,9 +306,9 @@ __global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, 
         const int ip = i / hnx, k = i - ip * hnx;
         double2 xa, xb;
         r2c_unpack(lds + ip * ld, k, mx, fac, &xa, &xb);
-        double2* o = Z2 + ((long)q * my + 2 * (rp0 + ip)) * hnx;
+        double2* o = Z2 + ((long)q * my + 2 * (rp0 + ip)) * zld;
         o[k] = xa;
-        o[hnx + k] = xb;
+        o[zld + k] = xb;
     }
 }
 
@@ -240,24 +318,36 @@ __global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, 
 // (source/transpose.cu:223-257).  Z2[g][q][s][kx] -> Z1[g][kx][q][ky].
 // -------------------------------------------------------------------------------------
 template <int FN = 0>
-__global__ __launch_bounds__(256) void k_span_fwd(const double2* __restrict__ Z2, long zstride, double2* __restrict__ Z1, long ostride, Dims d,
-                           int planes, FftPlan pl, int TK, int ld) {
+__global__ __launch_bounds__(256) void k_span_fwd(const double2* __restrict__ Z2, long zstride, int zld, double2* __restrict__ Z1,
+                           long ostride, Dims d, int planes, FftPlan pl, int TK, int ld) {
     RPCF_LDS(double2);
     const int g = blockIdx.z, q = blockIdx.y, kx0 = blockIdx.x * TK;
     const int nk = min(TK, d.hnx - kx0);
-    const int my = d.my, hnx = d.hnx, tid = threadIdx.x, nthr = blockDim.x;
+    const int my = FN ? FN : d.my, ny = d.ny, tid = threadIdx.x, nthr = blockDim.x;
     const double2* in = Z2 + g * zstride;
-    for (int i = tid; i < my * nk; i += nthr) {
-        const int s = i / nk, kxi = i - s * nk;
-        lds[kxi * ld + s] = in[((long)q * my + s) * hnx + kx0 + kxi];
-    }
+    staged<12>(
+        my * nk, tid, nthr,
+        [&](int i) {
+            const int s = i / nk, kxi = i - s * nk;
+            return in[((long)q * my + s) * zld + kx0 + kxi];
+        },
+        [&](int i, double2 v) {
+            const int s = i / nk, kxi = i - s * nk;
+            lds[kxi * ld + s] = v;
+        });
     __syncthreads();
-    fft_any<FN, -1>(lds, ld, nk, pl, tid, nthr);
+    fft_any<FN, -1, kSpanTile>(lds, ld, nk, pl, tid, nthr);
     double2* out = Z1 + g * ostride;
-    for (int i = tid; i < nk * d.ny; i += nthr) {
-        const int kxi = i / d.ny, ky = i - kxi * d.ny;
-        out[((long)(kx0 + kxi) * planes + q) * d.ny + ky] = lds[kxi * ld + span_of(d, ky)];
-    }
+    staged<8>(
+        nk * ny, tid, nthr,
+        [&](int i) {
+            const int kxi = i / ny, ky = i - kxi * ny;
+            return lds[kxi * ld + span_of(d, ky)];
+        },
+        [&](int i, double2 v) {
+            const int kxi = i / ny, ky = i - kxi * ny;
+            out[((long)(kx0 + kxi) * planes + q) * ny + ky] = v;
+        });
 }
 
 // Load one field's wall-normal values (P points, P = pz or nz) of TB pencils as the
@@ -293,23 +383,12 @@ __global__ __launch_bounds__(256) void k_cheb_p2s(const double2* __restrict__ Z1
     }
 }
 
-// Chebyshev derivative of one coefficient: ddz_sm (source/RPCFKernels.cu:476-520),
-// b_k = sum_{j=k+1,k+3,..<n} 2j a_j, b_0 halved.
-__device__ __forceinline__ double2 ddz_at(const double2* a, int k, int n) {
-    double re = 0.0, im = 0.0;
-    for (int j = k + 1; j < n; j += 2) {
-        const double dm = 2 * (double)j;
-        re = re + a[j].x * dm;
-        im = im + a[j].y * dm;
-    }
-    return k == 0 ? make_double2(re * 0.5, im * 0.5) : make_double2(re, im);
-}
-
 // -------------------------------------------------------------------------------------
 // Forward Chebyshev transform of the three Lamb components, rhsNonlinearKernel
 // (source/nonlinear.cu:146-242) and the unpadded s2p of N_v and N_eta
-// (source/rhs.cpp:96-97).  Tile: TB pencils (kx fixed).  Z1[3][hnx][pz][ny] ->
-// NLv, NLe [P][nz] (values at the nz collocation points; pencil (0,0) keeps L_x, L_y).
+// (source/rhs.cpp:96-97).  Tile: TB <= kChebTile pencils (kx fixed), 256 threads.
+// Z1[3][hnx][pz][ny] -> NLv, NLe [P][nz] (values at the nz collocation points; pencil
+// (0,0) keeps L_x, L_y).
 //
 // The reference's p2s is a length-4N FFT of the even extension u of the pz values
 // (cheby_pre_p2s/cheby_post_p2s, source/transform.cu:494-536).  Split by the parity of
@@ -317,7 +396,14 @@ __device__ __forceinline__ double2 ddz_at(const double2* a, int k, int n) {
 //   F_k = E_k + w4^k H_k,  E = FFT_2N(even extension of f_0, f_2, .., f_2N),
 //                          H = FFT_2N(f_1, f_3, .., f_{2N-1}, f_{2N-1}, .., f_1),
 // w4 = exp(-2 pi i/(4N)); a_k = F_k/(2N), a_0 halved; only k < nz is kept.
-// LDS: FFT tile TB*ld (ld >= 2N+1), then coefficient arrays Lx, Ly, Lz [TB][nz].
+//
+// Thread-owned elements: i = tid + u*256 <-> (pencil b = i / nz, coefficient k = i % nz);
+// the coefficients of the three components stay in registers.  The Chebyshev derivative
+// of ddz_sm (source/RPCFKernels.cu:476-520), D_k = sum_{j = k+1, k+3, .. <= N} 2j a_j with
+// D_0 halved, is a suffix sum over each parity class of j: two-level scan, 8 chunks per
+// (pencil, component, parity class), written in place over the coefficients of the other
+// class.  LDS: FFT tile [TB][ld] (ld >= 2N+2, also holding L_x, L_y as [TB][2][nz]) and
+// 256 chunk totals.
 // -------------------------------------------------------------------------------------
 template <int FN = 0>  // FN: compile-time 2N or 0
 __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, long zstride, double2* __restrict__ NLv,
@@ -325,91 +411,170 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, 
                                                 const double2* __restrict__ tw4, double aphi, double beta, int TB,
                                                 int ld) {
     RPCF_LDS(double2);
+    constexpr int KC = FN ? (kChebTile * (FN / 2 + 1) + 255) / 256 : (kChebTile * 257 + 255) / 256;
+    constexpr int CHM = FN ? (FN / 4 + 1 + 7) / 8 : (128 + 1 + 7) / 8;  // chunk length bound
     const int kx = blockIdx.y, ky0 = blockIdx.x * TB;
-    const int nb = min(TB, d.ny - ky0), nz = d.nz, N = d.N, ny = d.ny;
-    const int L = 2 * N;
+    const int nb = min(TB, d.ny - ky0);
+    const int N = FN ? FN / 2 : d.N, nz = N + 1, ny = d.ny, L = 2 * N;
     const int tid = threadIdx.x, nthr = blockDim.x;
     double2* fb = lds;
-    double2* coef = lds + TB * ld;  // [3][TB][nz]
+    double2* tot = lds + TB * ld;
     const double factor = (1.0 / (d.pz - 1));
-    for (int g = 0; g < 3; g++) {
+    const int total = nb * nz;
+    double2 ck[3][KC];
+    // one Lamb component g (compile-time, so that ck[g][.] stays in registers)
+    auto component = [&](auto gc) {
+        constexpr int g = decltype(gc)::value;
         const double2* in = Z1 + g * zstride + (long)kx * d.pz * ny + ky0;
         for (int half = 0; half < 2; half++) {
-            const int cnt = half ? N : N + 1;  // samples f_{2m+half}
-            for (int i = tid; i < cnt * nb; i += nthr) {
-                const int m = i / nb, b = i - m * nb;
-                const double2 v = in[(long)(2 * m + half) * ny + b];
-                double2* sq = fb + b * ld;
-                if (half == 0) {
+            // samples f_{2m+half}: row 2m+half of the plane stack, nb consecutive ky
+            staged<KC>(
+                (half ? N : N + 1) * nb, tid, nthr,
+                [&](int i) {
+                    const int m = i / nb, b = i - m * nb;
+                    return in[(long)(2 * m + half) * ny + b];
+                },
+                [&](int i, double2 v) {
+                    const int m = i / nb, b = i - m * nb;
+                    double2* sq = fb + b * ld;
                     sq[m] = v;
-                    if (m >= 1 && m <= N - 1) sq[L - m] = v;
-                } else {
-                    sq[m] = v;
-                    sq[L - 1 - m] = v;
-                }
-            }
+                    if (half == 0) {
+                        if (m >= 1 && m <= N - 1) sq[L - m] = v;
+                    } else {
+                        sq[L - 1 - m] = v;
+                    }
+                });
             __syncthreads();
-            fft_any<FN, -1>(fb, ld, nb, plN, tid, nthr);
-            for (int i = tid; i < nb * nz; i += nthr) {
-                const int b = i / nz, k = i - b * nz;
-                double2* c = coef + (g * TB + b) * nz + k;
-                if (half == 0) {
-                    *c = fb[b * ld + k];
-                } else {
-                    double2 t = cscale(cadd(*c, cmul(tw4[k], fb[b * ld + k])), factor);
-                    if (k == 0) t = cscale(t, 0.5);
-                    *c = t;
+            fft_any<FN, -1, kChebTile>(fb, ld, nb, plN, tid, nthr);
+#pragma unroll
+            for (int u = 0; u < KC; u++) {
+                const int i = tid + u * nthr;
+                if (i < total) {
+                    const int b = i / nz, k = i - b * nz;
+                    const double2 x = fb[b * ld + k];
+                    if (half == 0) {
+                        ck[g][u] = x;
+                    } else {
+                        double2 t = cscale(cadd(ck[g][u], cmul(tw4[k], x)), factor);
+                        if (k == 0) t = cscale(t, 0.5);
+                        ck[g][u] = t;
+                    }
                 }
             }
             __syncthreads();
         }
-    }
-    // rhsNonlinear: N_v -> the Lz slot (read only by its own thread), N_eta -> the FFT tile
-    for (int i = tid; i < nb * nz; i += nthr) {
-        const int b = i / nz, k = i - b * nz, ky = ky0 + b;
-        const double2* Lx = coef + (0 * TB + b) * nz;
-        const double2* Ly = coef + (1 * TB + b) * nz;
-        double2* Lz = coef + (2 * TB + b) * nz;
-        double2 nv, ne;
-        if (kx == 0 && ky == 0) {
-            nv = Lx[k];
-            ne = Ly[k];
-        } else {
-            const double ia = (double)kx / aphi;
-            const double ib = ky >= ny / 2 + 1 ? (double)(ky - ny) / beta : (double)ky / beta;
-            const double kmn = ia * ia + ib * ib;
-            const double2 dx = ddz_at(Lx, k, nz), dy = ddz_at(Ly, k, nz);
-            double ure = kmn * Lz[k].x - ia * dx.y;
-            double uim = kmn * Lz[k].y + ia * dx.x;
-            ure = ure - ib * dy.y;
-            uim = uim + ib * dy.x;
-            const double wre = -1.0 * ia * Ly[k].y + ib * Lx[k].y;
-            const double wim = ia * Ly[k].x - ib * Lx[k].x;
-            nv = make_double2(ure, uim);
-            ne = make_double2(wre, wim);
-        }
-        Lz[k] = nv;
-        fb[b * ld + k] = ne;
-    }
-    __syncthreads();
-    for (int i = tid; i < nb * nz; i += nthr) {
-        const int b = i / nz, k = i - b * nz;
-        coef[(0 * TB + b) * nz + k] = fb[b * ld + k];  // N_eta -> the Lx slot
-    }
-    __syncthreads();
-    for (int h = 0; h < 2; h++) {
-        const double2* src = coef + (h ? 0 : 2) * TB * nz;  // N_v in the Lz slot, N_eta in the Lx slot
-        for (int i = tid; i < nb * nz; i += nthr) {
+    };
+    component(IntC<0>{});
+    component(IntC<1>{});
+    component(IntC<2>{});
+    // L_x, L_y -> LDS [b][0..1][nz]
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
             const int b = i / nz, k = i - b * nz;
-            s2p_even_ext(fb + b * ld, src[b * nz + k], k, N);
+            fb[b * ld + k] = ck[0][u];
+            fb[b * ld + nz + k] = ck[1][u];
+        }
+    }
+    __syncthreads();
+    {
+        // scan thread: pencil b, component c2 (0: L_x, 1: L_y), class par (j = 2t + par), chunk c
+        const int b = tid >> 5, c2 = (tid >> 4) & 1, par = (tid >> 3) & 1, c = tid & 7;
+        const int Lc = (N - par) / 2 + 1;  // class size: j = par, par+2, .. <= N
+        const int CH = (Lc + 7) / 8;
+        const int t0 = c * CH, t1 = min(t0 + CH, Lc);
+        double2* A = fb + b * ld + c2 * nz;
+        double2 cj[CHM];
+        double sre = 0.0, sim = 0.0;
+        if (b < nb) {
+#pragma unroll
+            for (int e = 0; e < CHM; e++) {
+                const int t = t0 + e;
+                if (t < t1) {
+                    const int j = 2 * t + par;
+                    const double dm = 2 * (double)j;
+                    cj[e] = make_double2(A[j].x * dm, A[j].y * dm);
+                    sre += cj[e].x;
+                    sim += cj[e].y;
+                }
+            }
+        }
+        tot[tid] = make_double2(sre, sim);
+        __syncthreads();
+        if (b < nb) {
+            double rre = 0.0, rim = 0.0;
+            for (int cc = 7; cc > c; cc--) {
+                const double2 o = tot[(tid & ~7) | cc];
+                rre += o.x;
+                rim += o.y;
+            }
+#pragma unroll
+            for (int e = CHM - 1; e >= 0; e--) {
+                const int t = t0 + e;
+                if (t < t1) {
+                    const int j = 2 * t + par;
+                    rre += cj[e].x;
+                    rim += cj[e].y;
+                    if (j >= 1) A[j - 1] = j == 1 ? make_double2(rre * 0.5, rim * 0.5) : make_double2(rre, rim);
+                }
+            }
+            if (c == 0 && par == (N & 1)) A[N] = make_double2(0.0, 0.0);  // no j > N
+        }
+    }
+    __syncthreads();
+    // rhsNonlinear: N_v, N_eta per (pencil, k) into the registers of L_x, L_y
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz, ky = ky0 + b;
+            const double2 lx = ck[0][u], ly = ck[1][u], lz = ck[2][u];
+            double2 nv, ne;
+            if (kx == 0 && ky == 0) {
+                nv = lx;
+                ne = ly;
+            } else {
+                const double ia = (double)kx / aphi;
+                const double ib = ky >= ny / 2 + 1 ? (double)(ky - ny) / beta : (double)ky / beta;
+                const double kmn = ia * ia + ib * ib;
+                const double2 dx = fb[b * ld + k], dy = fb[b * ld + nz + k];
+                double ure = kmn * lz.x - ia * dx.y;
+                double uim = kmn * lz.y + ia * dx.x;
+                ure = ure - ib * dy.y;
+                uim = uim + ib * dy.x;
+                const double wre = -1.0 * ia * ly.y + ib * lx.y;
+                const double wim = ia * ly.x - ib * lx.x;
+                nv = make_double2(ure, uim);
+                ne = make_double2(wre, wim);
+            }
+            ck[0][u] = nv;
+            ck[1][u] = ne;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int u = 0; u < KC; u++) {
+            const int i = tid + u * nthr;
+            if (i < total) {
+                const int b = i / nz, k = i - b * nz;
+                s2p_even_ext(fb + b * ld, ck[h][u], k, N);
+            }
         }
         __syncthreads();
-        fft_any<FN, -1>(fb, ld, nb, plN, tid, nthr);
+        fft_any<FN, -1, kChebTile>(fb, ld, nb, plN, tid, nthr);
         double2* out = h ? NLe : NLv;
-        for (int i = tid; i < nb * nz; i += nthr) {
-            const int b = i / nz, p = i - b * nz;
-            out[((long)(ky0 + b) * d.hnx + kx) * nz + p] = fb[b * ld + p];
-        }
+        staged<8>(
+            total, tid, nthr, [&](int i) {
+                const int b = i / nz, p = i - b * nz;
+                return fb[b * ld + p];
+            },
+            [&](int i, double2 v) {
+                const int b = i / nz, p = i - b * nz;
+                out[((long)(ky0 + b) * d.hnx + kx) * nz + p] = v;
+            });
         __syncthreads();
     }
 }

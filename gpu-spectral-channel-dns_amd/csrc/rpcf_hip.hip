@@ -53,7 +53,8 @@ struct rpcf_ctx {
     double2* nl[4] = {nullptr, nullptr, nullptr, nullptr};  // N_v, N_v prev, N_eta, N_eta prev
     double2* Y = nullptr;   // [6][hnx][pz][ny] (also Z1 [3][hnx][pz][ny])
     double2* W = nullptr;   // [6][pz][my][nxh]
-    double2* Z2 = nullptr;  // [3][pz][my][hnx]
+    double2* Z2 = nullptr;  // [3][pz][my][zld], zld = hnx rounded up to 8 (128-byte rows)
+    int zld = 0;
     double2* minv = nullptr;
     double2* Wlin = nullptr;  // [5][P][nz] operator products of the linear right-hand sides
     double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *U = nullptr, *dU = nullptr, *ddU = nullptr;
@@ -229,7 +230,7 @@ int enq_nonlinear(rpcf_ctx* c) {
     const Dims d = c->d;
     int rc;
     const long fS = (long)d.P * d.nz, yS = (long)d.hnx * d.pz * d.ny, wS = (long)d.pz * d.my * d.nxh,
-               zS = (long)d.pz * d.my * d.hnx;
+               zS = (long)d.pz * d.my * c->zld;
     const double2* tw4 = (const double2*)c->pl_zp.tw;
     {
         const int TB = c->TB_s2p, ld = c->ld_zn;
@@ -265,10 +266,10 @@ int enq_nonlinear(rpcf_ctx* c) {
         size_t lds = sizeof(double2) * 6 * RP * ld;
         rc = launch(c, KID_LAMB, [&] {
             switch (d.mx) {
-#define RPCF_CASE(L) case L: k_lamb_phys<L><<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->meanU, c->Ro, c->fac, d, c->pl_x, RP, ld); break;
+#define RPCF_CASE(L) case L: k_lamb_phys<L><<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->zld, c->meanU, c->Ro, c->fac, d, c->pl_x, RP, ld); break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-                default: k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->meanU, c->Ro, c->fac, d,
+                default: k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->zld, c->meanU, c->Ro, c->fac, d,
                                                                          c->pl_x, RP, ld);
             }
         });
@@ -280,10 +281,10 @@ int enq_nonlinear(rpcf_ctx* c) {
         size_t lds = sizeof(double2) * TK * ld;
         rc = launch(c, KID_SPAN_FWD, [&] {
             switch (d.my) {
-#define RPCF_CASE(L) case L: k_span_fwd<L><<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->Y, yS, d, d.pz, c->pl_y, TK, ld); break;
+#define RPCF_CASE(L) case L: k_span_fwd<L><<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->zld, c->Y, yS, d, d.pz, c->pl_y, TK, ld); break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-                default: k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->Y, yS, d, d.pz, c->pl_y, TK, ld);
+                default: k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->zld, c->Y, yS, d, d.pz, c->pl_y, TK, ld);
             }
         });
         if (rc) return rc;
@@ -294,7 +295,7 @@ int enq_nonlinear(rpcf_ctx* c) {
     {
         const int TB = c->TB_nl, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx);
-        size_t lds = sizeof(double2) * ((size_t)TB * ld + 3 * (size_t)TB * d.nz);
+        size_t lds = sizeof(double2) * ((size_t)TB * ld + 256);
         rc = launch(c, KID_P2S_NL, [&] {
             switch (2 * d.N) {
 #define RPCF_CASE(L) case L: k_p2s_nl<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->nl[0], c->nl[2], d, c->pl_zn, tw4, c->lp.aphi, c->lp.beta, TB, ld); break;
@@ -403,7 +404,8 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
         if ((rc = dalloc(c, &c->nl[i], P * nz))) return rc;
     if ((rc = dalloc(c, &c->Y, 6 * (size_t)d.hnx * d.pz * d.ny))) return rc;
     if ((rc = dalloc(c, &c->W, 6 * (size_t)d.pz * d.my * d.nxh))) return rc;
-    if ((rc = dalloc(c, &c->Z2, 3 * (size_t)d.pz * d.my * d.hnx))) return rc;
+    c->zld = (d.hnx + 7) & ~7;
+    if ((rc = dalloc(c, &c->Z2, 3 * (size_t)d.pz * d.my * c->zld))) return rc;
     if ((rc = dalloc(c, &c->minv, 2 * (size_t)d.nslot * nz * nz))) return rc;
     if ((rc = dalloc(c, &c->Wlin, 5 * P * nz))) return rc;
     if ((rc = dalloc(c, &c->T0, nz * nz)) || (rc = dalloc(c, &c->T2, nz * nz)) || (rc = dalloc(c, &c->T4, nz * nz)))
@@ -467,16 +469,16 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     lim = c->lds_limit;
     // Chebyshev kernels: tiles sized for three resident blocks per CU
     const int lim3 = lim / 3;
-    c->ld_zn = 2 * d.N + 1;
+    c->ld_zn = 2 * d.N + 2;  // >= 2N (FFT) and >= 2 nz (L_x, L_y of k_p2s_nl)
     c->ld_my = d.my + 1;
     c->ld_mx = d.mx + 1;
-    c->TB_s2p = pick_tile(16, 2 * d.N, c->cap_zn, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_zn; });
-    c->TB_nl = pick_tile(16, 2 * d.N, c->cap_zn, lim3, [&](int t) {
-        return sizeof(double2) * ((size_t)t * c->ld_zn + 3 * (size_t)t * d.nz);
+    c->TB_s2p = pick_tile(kChebTile, 2 * d.N, c->cap_zn, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_zn; });
+    c->TB_nl = pick_tile(kChebTile, 2 * d.N, c->cap_zn, lim3, [&](int t) {
+        return sizeof(double2) * ((size_t)t * c->ld_zn + 256);
     });
-    c->TK_inv = pick_tile(16, d.my, c->cap_y, lim, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_my; });
+    c->TK_inv = pick_tile(kSpanTile, d.my, c->cap_y, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_my; });
     c->TK_fwd = c->TK_inv;
-    c->RP = pick_tile(2, 6 * d.mx, c->cap_x, lim, [&](int t) { return sizeof(double2) * 6 * (size_t)t * c->ld_mx; });
+    c->RP = pick_tile(kLambPairs, 6 * d.mx, c->cap_x, lim3, [&](int t) { return sizeof(double2) * 6 * (size_t)t * c->ld_mx; });
     if (!c->TB_s2p || !c->TB_nl || !c->TK_inv || !c->RP)
         return fail(c, RPCF_EINVAL, "grid too large for the LDS tiles (limit %d bytes)", lim);
     allow_lds(k_cheb_s2p<1>, lim);
@@ -699,10 +701,10 @@ int rpcf_phys_to_spec(rpcf_ctx* c, const double* phys, double* spec, long stride
         rc = fail(c, RPCF_EDEVICE, "phys_to_spec copy");
     if (!rc) {
         dim3 g3((d.my / 2 + RP1 - 1) / RP1, planes);
-        k_x_r2c<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(ph, z, c->fac, d, c->pl_x, RP1,
+        k_x_r2c<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(ph, z, d.hnx, c->fac, d, c->pl_x, RP1,
                                                                                   c->ld_mx);
         dim3 g2((d.hnx + c->TK_fwd - 1) / c->TK_fwd, planes, 1);
-        k_span_fwd<0><<<g2, kFftThreads, sizeof(double2) * c->TK_fwd * c->ld_my, c->stream>>>(z, 0, y, 0, d, planes,
+        k_span_fwd<0><<<g2, kFftThreads, sizeof(double2) * c->TK_fwd * c->ld_my, c->stream>>>(z, 0, d.hnx, y, 0, d, planes,
                                                                                            c->pl_y, c->TK_fwd, c->ld_my);
         dim3 g((d.ny + TB - 1) / TB, d.hnx);
         k_cheb_p2s<<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(y, f, d, planes, plz, TB, ld);
