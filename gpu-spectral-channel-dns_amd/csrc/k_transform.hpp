@@ -224,18 +224,44 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W
     const int q = blockIdx.y, rp0 = blockIdx.x * RP;
     const int nrp = min(RP, d.my / 2 - rp0);
     const int mx = FN ? FN : d.mx, my = d.my, nxh = d.nxh, tid = threadIdx.x, nthr = blockDim.x;
-    staged<9>(
-        6 * nrp * mx, tid, nthr,
-        [&](int i) {
-            const int seq = i / mx, k = i - seq * mx;
-            const int f = seq / nrp, ip = seq - f * nrp;
-            const double2* rA = W + f * wstride + ((long)q * my + 2 * (rp0 + ip)) * nxh;
-            return c2r_pack(rA, rA + nxh, k, mx, nxh);
-        },
-        [&](int i, double2 v) {
-            const int seq = i / mx, k = i - seq * mx;
-            lds[seq * ld + k] = v;
-        });
+    // packed spectra Z = X_A + i X_B of the row pairs (c2r_pack, element by element): each
+    // kept coefficient pair (A_k, B_k), k < nxh, is loaded once and gives Z_k and
+    // Z_{mx-k}; Z_k = 0 for nxh <= k <= mx - nxh
+    constexpr int KL = 3;  // loads per thread in flight: 6 * kLambPairs * 64 / 256
+    for (int base = tid; base < 6 * nrp * nxh; base += KL * nthr) {
+        double2 va[KL], vb[KL];
+#pragma unroll
+        for (int u = 0; u < KL; u++) {
+            const int i = base + u * nthr;
+            if (i < 6 * nrp * nxh) {
+                const int seq = i / nxh, k = i - seq * nxh;
+                const int f = seq / nrp, ip = seq - f * nrp;
+                const double2* rA = W + f * wstride + ((long)q * my + 2 * (rp0 + ip)) * nxh;
+                va[u] = rA[k];
+                vb[u] = rA[nxh + k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KL; u++) {
+            const int i = base + u * nthr;
+            if (i < 6 * nrp * nxh) {
+                const int seq = i / nxh, k = i - seq * nxh;
+                double2 a = va[u], b = vb[u];
+                double2* z = lds + seq * ld;
+                if (k == 0) {
+                    a.y = 0.0;
+                    b.y = 0.0;
+                } else {
+                    z[mx - k] = make_double2(a.x + b.y, -a.y + b.x);
+                }
+                z[k] = make_double2(a.x - b.y, a.y + b.x);
+            }
+        }
+    }
+    for (int i = tid; i < 6 * nrp * (mx - 2 * nxh + 1); i += nthr) {
+        const int seq = i / (mx - 2 * nxh + 1), k = i - seq * (mx - 2 * nxh + 1);
+        lds[seq * ld + nxh + k] = make_double2(0.0, 0.0);
+    }
     __syncthreads();
     fft_any<FN, +1, 6 * kLambPairs>(lds, ld, 6 * nrp, pl, tid, nthr);
     const double mU = meanU[q];
@@ -383,6 +409,57 @@ __global__ __launch_bounds__(256) void k_cheb_p2s(const double2* __restrict__ Z1
     }
 }
 
+// Chebyshev derivative of ddz_sm (source/RPCFKernels.cu:476-520), D_k = sum_{j = k+1,
+// k+3, .. <= N} 2j a_j with D_0 halved, of the two coefficient arrays A_c = fb + b*ld + c*nz
+// (c = 0, 1) of each of the nb <= 8 pencils of a tile, in place.  D is a suffix sum over
+// each parity class of j: two-level scan with 8 chunks per (pencil, array, class) on 256
+// threads, written over the coefficients of the other class (read before the barrier).
+// Callers synchronise before (arrays written) and after (D read).  tot: 256 LDS entries.
+template <int CHM>  // chunk length bound: >= ceil((N/2 + 1) / 8)
+__device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int N, int nb, double2* tot, int tid) {
+    const int b = tid >> 5, c2 = (tid >> 4) & 1, par = (tid >> 3) & 1, c = tid & 7;
+    const int Lc = (N - par) / 2 + 1;  // class size: j = par, par+2, .. <= N
+    const int CH = (Lc + 7) / 8;
+    const int t0 = c * CH, t1 = min(t0 + CH, Lc);
+    double2* A = fb + b * ld + c2 * nz;
+    double2 cj[CHM];
+    double sre = 0.0, sim = 0.0;
+    if (b < nb) {
+#pragma unroll
+        for (int e = 0; e < CHM; e++) {
+            const int t = t0 + e;
+            if (t < t1) {
+                const int j = 2 * t + par;
+                const double dm = 2 * (double)j;
+                cj[e] = make_double2(A[j].x * dm, A[j].y * dm);
+                sre += cj[e].x;
+                sim += cj[e].y;
+            }
+        }
+    }
+    tot[tid] = make_double2(sre, sim);
+    __syncthreads();
+    if (b < nb) {
+        double rre = 0.0, rim = 0.0;
+        for (int cc = 7; cc > c; cc--) {
+            const double2 o = tot[(tid & ~7) | cc];
+            rre += o.x;
+            rim += o.y;
+        }
+#pragma unroll
+        for (int e = CHM - 1; e >= 0; e--) {
+            const int t = t0 + e;
+            if (t < t1) {
+                const int j = 2 * t + par;
+                rre += cj[e].x;
+                rim += cj[e].y;
+                if (j >= 1) A[j - 1] = j == 1 ? make_double2(rre * 0.5, rim * 0.5) : make_double2(rre, rim);
+            }
+        }
+        if (c == 0 && par == (N & 1)) A[N] = make_double2(0.0, 0.0);  // no j > N
+    }
+}
+
 // -------------------------------------------------------------------------------------
 // Forward Chebyshev transform of the three Lamb components, rhsNonlinearKernel
 // (source/nonlinear.cu:146-242) and the unpadded s2p of N_v and N_eta
@@ -398,12 +475,9 @@ __global__ __launch_bounds__(256) void k_cheb_p2s(const double2* __restrict__ Z1
 // w4 = exp(-2 pi i/(4N)); a_k = F_k/(2N), a_0 halved; only k < nz is kept.
 //
 // Thread-owned elements: i = tid + u*256 <-> (pencil b = i / nz, coefficient k = i % nz);
-// the coefficients of the three components stay in registers.  The Chebyshev derivative
-// of ddz_sm (source/RPCFKernels.cu:476-520), D_k = sum_{j = k+1, k+3, .. <= N} 2j a_j with
-// D_0 halved, is a suffix sum over each parity class of j: two-level scan, 8 chunks per
-// (pencil, component, parity class), written in place over the coefficients of the other
-// class.  LDS: FFT tile [TB][ld] (ld >= 2N+2, also holding L_x, L_y as [TB][2][nz]) and
-// 256 chunk totals.
+// the coefficients of the three components stay in registers; the derivatives of L_x and
+// L_y come from cheb_deriv_scan.  LDS: FFT tile [TB][ld] (ld >= 2N+2, also holding L_x, L_y
+// as [TB][2][nz]) and 256 chunk totals.
 // -------------------------------------------------------------------------------------
 template <int FN = 0>  // FN: compile-time 2N or 0
 __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, long zstride, double2* __restrict__ NLv,
@@ -478,50 +552,7 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, 
         }
     }
     __syncthreads();
-    {
-        // scan thread: pencil b, component c2 (0: L_x, 1: L_y), class par (j = 2t + par), chunk c
-        const int b = tid >> 5, c2 = (tid >> 4) & 1, par = (tid >> 3) & 1, c = tid & 7;
-        const int Lc = (N - par) / 2 + 1;  // class size: j = par, par+2, .. <= N
-        const int CH = (Lc + 7) / 8;
-        const int t0 = c * CH, t1 = min(t0 + CH, Lc);
-        double2* A = fb + b * ld + c2 * nz;
-        double2 cj[CHM];
-        double sre = 0.0, sim = 0.0;
-        if (b < nb) {
-#pragma unroll
-            for (int e = 0; e < CHM; e++) {
-                const int t = t0 + e;
-                if (t < t1) {
-                    const int j = 2 * t + par;
-                    const double dm = 2 * (double)j;
-                    cj[e] = make_double2(A[j].x * dm, A[j].y * dm);
-                    sre += cj[e].x;
-                    sim += cj[e].y;
-                }
-            }
-        }
-        tot[tid] = make_double2(sre, sim);
-        __syncthreads();
-        if (b < nb) {
-            double rre = 0.0, rim = 0.0;
-            for (int cc = 7; cc > c; cc--) {
-                const double2 o = tot[(tid & ~7) | cc];
-                rre += o.x;
-                rim += o.y;
-            }
-#pragma unroll
-            for (int e = CHM - 1; e >= 0; e--) {
-                const int t = t0 + e;
-                if (t < t1) {
-                    const int j = 2 * t + par;
-                    rre += cj[e].x;
-                    rim += cj[e].y;
-                    if (j >= 1) A[j - 1] = j == 1 ? make_double2(rre * 0.5, rim * 0.5) : make_double2(rre, rim);
-                }
-            }
-            if (c == 0 && par == (N & 1)) A[N] = make_double2(0.0, 0.0);  // no j > N
-        }
-    }
+    cheb_deriv_scan<CHM>(fb, ld, nz, N, nb, tot, tid);
     __syncthreads();
     // rhsNonlinear: N_v, N_eta per (pencil, k) into the registers of L_x, L_y
 #pragma unroll

@@ -33,6 +33,8 @@ struct Pending {
 
 }  // namespace
 
+enum { FIELDS_STALE = 0, FIELDS_FRESH = 1, FIELDS_EXPLICIT = 2 };
+
 struct rpcf_ctx {
     rpcf_params prm{};
     Dims d{};
@@ -60,10 +62,16 @@ struct rpcf_ctx {
     double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *U = nullptr, *dU = nullptr, *ddU = nullptr;
     // transposed tables for k_linear: T0, T2, and the quotients T4*dt*0.5/Re, T2*0.5/Re*dt
     double *T0t = nullptr, *T2t = nullptr, *T4q = nullptr, *T2q = nullptr;
+    double* Tf = nullptr;  // [5][RT][S][64] A-operand tiles of k_lin_tv_mfma
+    int tv_rt = 0, tv_s = 0;
     double *meanU = nullptr, *qw = nullptr, *tw = nullptr;
     std::vector<double> qw_host;
     double2 q_target = make_double2(0.0, 0.0);
     int have_state = 0;
+    // the six spectral fields F relative to the state: FIELDS_STALE = velocity(state), not
+    // computed yet; FIELDS_FRESH = computed from the current state; FIELDS_EXPLICIT = set by
+    // the caller, or held over a linear step, so not derivable from the state
+    int fields = 0;
     // per-kernel timing with hipEvents on the context stream
     int timing = 0;
     std::vector<Pending> pending;
@@ -232,6 +240,10 @@ int enq_nonlinear(rpcf_ctx* c) {
     const long fS = (long)d.P * d.nz, yS = (long)d.hnx * d.pz * d.ny, wS = (long)d.pz * d.my * d.nxh,
                zS = (long)d.pz * d.my * c->zld;
     const double2* tw4 = (const double2*)c->pl_zp.tw;
+    if (c->fields == FIELDS_STALE) {
+        if ((rc = enq_velocity(c))) return rc;
+        c->fields = FIELDS_FRESH;
+    }
     {
         const int TB = c->TB_s2p, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, 6);
@@ -319,10 +331,22 @@ int enq_linear(rpcf_ctx* c) {
     st.qw_target_re = c->q_target.x; st.qw_target_im = c->q_target.y;
     st.flow_control = c->prm.flow_control;
     int rc = launch(c, KID_LIN_TV, [&] {
-        const int g = (d.P + kTvPencils - 1) / kTvPencils;
-        const int thr = 384;
-        const size_t lds = sizeof(double2) * 2 * (size_t)d.nz * (kTvPencils + 1);
-        k_lin_tv<kTvPencils><<<g, thr, lds, c->stream>>>(c->sv, c->se, c->Wlin, d, c->T0t, c->T2t, c->T4q, c->T2q);
+        dim3 g((d.P + kTvBlockPencils - 1) / kTvBlockPencils, 5);
+        const size_t lds = sizeof(double) * kTvChunk * c->tv_s * 64;
+        switch (2 * d.N) {
+#define RPCF_CASE(L)                                                                                    \
+    case L:                                                                                             \
+        k_lin_tv_mfma<(L / 2 + 4) / 4><<<g, 256, lds, c->stream>>>(c->sv, c->se, c->Wlin, d, c->Tf, c->tv_rt); \
+        break;
+            RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+            default: {
+                const int gv = (d.P + kTvPencils - 1) / kTvPencils;
+                const size_t lv = sizeof(double2) * 2 * (size_t)d.nz * (kTvPencils + 1);
+                k_lin_tv<kTvPencils><<<gv, 384, lv, c->stream>>>(c->sv, c->se, c->Wlin, d, c->T0t, c->T2t, c->T4q,
+                                                                 c->T2q);
+            }
+        }
     });
     if (rc) return rc;
     return launch(c, KID_LINEAR, [&] {
@@ -335,7 +359,8 @@ void free_ctx(rpcf_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->Wlin, c->F, c->sv, c->se, c->nl[0], c->nl[1], c->nl[2], c->nl[3], c->Y, c->W, c->Z2, c->minv,
-                    c->T0, c->T2, c->T4, c->U, c->dU, c->ddU, c->meanU, c->qw, c->tw, c->T0t, c->T2t, c->T4q, c->T2q};
+                    c->T0, c->T2, c->T4, c->U, c->dU, c->ddU, c->meanU, c->qw, c->tw, c->T0t, c->T2t, c->T4q, c->T2q,
+                    c->Tf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -448,6 +473,21 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
         HIPCHK(c, hipMemcpy(c->T2t, t2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->T4q, q4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->T2q, q2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        // the five operators (T0, T2, T4q for v; T0, T2q for eta) as MFMA A-operand tiles
+        const int RT = (int)((nz + 15) / 16), S = (int)((nz + 3) / 4);
+        c->tv_rt = RT;
+        c->tv_s = S;
+        const std::vector<double>* ops[5] = {&t0, &t2, &q4, &t0, &q2};  // transposed: op[j*nz + i] = T[i][j]
+        std::vector<double> tf((size_t)5 * RT * S * 64, 0.0);
+        for (int m = 0; m < 5; m++)
+            for (int rt = 0; rt < RT; rt++)
+                for (int sk = 0; sk < S; sk++)
+                    for (int l = 0; l < 64; l++) {
+                        const size_t i = 16 * rt + (l & 15), k = 4 * sk + (l >> 4);
+                        if (i < nz && k < nz) tf[(((size_t)m * RT + rt) * S + sk) * 64 + l] = (*ops[m])[k * nz + i];
+                    }
+        if ((rc = dalloc(c, &c->Tf, tf.size()))) return rc;
+        HIPCHK(c, hipMemcpy(c->Tf, tf.data(), sizeof(double) * tf.size(), hipMemcpyHostToDevice));
     }
     HIPCHK(c, hipMemcpy(c->U, U.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dU, dU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
@@ -493,7 +533,8 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     allow_lds(k_span_inv<L>, lim);    \
     allow_lds(k_span_fwd<L>, lim);    \
     allow_lds(k_lamb_phys<L>, lim);   \
-    allow_lds(k_p2s_nl<L>, lim);
+    allow_lds(k_p2s_nl<L>, lim);      \
+    allow_lds(k_lin_tv_mfma<(L / 2 + 4) / 4>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     allow_lds(k_cheb_p2s, lim);
@@ -566,9 +607,8 @@ int rpcf_set_state(rpcf_ctx* c, const double* v_hat, const double* eta_hat, long
         q = make_double2(q.x + s[0] * c->qw_host[k], q.y + s[1] * c->qw_host[k]);
     }
     c->q_target = q;
-    rc = enq_velocity(c);
-    if (rc) return rc;
-    return sync_ctx(c);
+    c->fields = FIELDS_STALE;
+    return RPCF_OK;
 }
 
 int rpcf_get_state(rpcf_ctx* c, double* v_hat, double* eta_hat, long stride) {
@@ -580,14 +620,27 @@ int rpcf_get_state(rpcf_ctx* c, double* v_hat, double* eta_hat, long stride) {
 int rpcf_set_field(rpcf_ctx* c, int field, const double* f_hat, long stride) {
     if (!c) return RPCF_EINVAL;
     if (field < 0 || field > 5) return fail(c, RPCF_EINVAL, "bad field id %d", field);
+    if (c->fields == FIELDS_STALE && c->have_state) {
+        // the other five fields keep their values: materialise them first
+        int rc = enq_velocity(c);
+        if (rc) return rc;
+    }
     int rc = upload_spec(c, c->F + (size_t)field * spec_elems(c), f_hat, stride);
-    if (!rc) c->have_state = 1;
+    if (!rc) {
+        c->have_state = 1;
+        c->fields = FIELDS_EXPLICIT;
+    }
     return rc;
 }
 
 int rpcf_get_field(rpcf_ctx* c, int field, double* f_hat, long stride) {
     if (!c) return RPCF_EINVAL;
     if (field < 0 || field > 5) return fail(c, RPCF_EINVAL, "bad field id %d", field);
+    if (c->fields == FIELDS_STALE && c->have_state) {
+        int rc = enq_velocity(c);
+        if (rc) return rc;
+        c->fields = FIELDS_FRESH;
+    }
     return download_spec(c, f_hat, c->F + (size_t)field * spec_elems(c), stride);
 }
 
@@ -613,7 +666,13 @@ int rpcf_nonlinear(rpcf_ctx* c) {
 int rpcf_linear(rpcf_ctx* c) {
     if (!c) return RPCF_EINVAL;
     if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_linear: no state set");
+    // the fields keep the velocity of the state before the solve until rpcf_velocity
+    if (c->fields == FIELDS_STALE) {
+        int rc = enq_velocity(c);
+        if (rc) return rc;
+    }
     int rc = enq_linear(c);
+    c->fields = FIELDS_EXPLICIT;
     return rc ? rc : sync_ctx(c);
 }
 
@@ -621,17 +680,20 @@ int rpcf_velocity(rpcf_ctx* c) {
     if (!c) return RPCF_EINVAL;
     if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_velocity: no state set");
     int rc = enq_velocity(c);
+    c->fields = FIELDS_FRESH;
     return rc ? rc : sync_ctx(c);
 }
 
 int rpcf_step(rpcf_ctx* c, int nsteps) {
     if (!c) return RPCF_EINVAL;
     if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_step: no state set");
+    // nextStep + getUVW: the velocity of the new state is formed when it is next needed
+    // (by the next nonlinear evaluation, or by rpcf_get_field)
     for (int s = 0; s < nsteps; s++) {
         int rc = enq_nonlinear(c);
         if (!rc) rc = enq_linear(c);
-        if (!rc) rc = enq_velocity(c);
         if (rc) return rc;
+        c->fields = FIELDS_STALE;
     }
     return sync_ctx(c);
 }
