@@ -172,12 +172,22 @@ __device__ __forceinline__ double2 zddz(const double2* a, int k, int n) {
 
 // -------------------------------------------------------------------------------------
 // Velocity recovery: getVelocityKernel_sm (source/velocity.cu:90-170) after the state
-// copy of source/solver.cpp:95-96.  One block per pencil, thread k = coefficient k.
-// F[6][P][nz] <- (u, v, w, ox, oy, oz) from (v-hat, eta-hat).
+// copy of source/solver.cpp:95-96.  One block per pencil (x) and member (y), thread k =
+// coefficient k.  F[6][P][nz] <- (u, v, w, ox, oy, oz) from (v-hat, eta-hat).
 // -------------------------------------------------------------------------------------
-__global__ void k_velocity(const double2* __restrict__ sv, const double2* __restrict__ se, double2* __restrict__ F,
-                           long fstride, Dims d, LinParams lp) {
+__global__ void k_velocity(const double2* __restrict__ sv, const double2* __restrict__ se, long sS,
+                           double2* __restrict__ F, long fstride, long sF, Dims d, Members mb) {
     __shared__ double2 sw[256], soz[256], su[256], sv2[256];
+    const int slot = mb.ids[blockIdx.y];
+    const MemberParams& mp = mb.mp[slot];
+    LinParams lp;
+    lp.aphi = mp.aphi;
+    lp.beta = mp.beta;
+    lp.Re = mp.Re;
+    lp.dt = mp.dt;
+    sv += slot * sS;
+    se += slot * sS;
+    F += slot * sF;
     const int p = blockIdx.x, kx = p % d.hnx, ky = p / d.hnx, k = threadIdx.x, nz = d.nz;
     const long o = (long)p * nz + k;
     if (k < nz) {
@@ -274,10 +284,18 @@ struct LinState {
 constexpr int kTvRows = 4, kTvCols = 8;
 
 template <int NPB>
-__global__ __launch_bounds__(384) void k_lin_tv(const double2* __restrict__ sv, const double2* __restrict__ se,
-                                                double2* __restrict__ W, Dims d, const double* __restrict__ T0t,
+__global__ __launch_bounds__(384) void k_lin_tv(const double2* __restrict__ sv, const double2* __restrict__ se, long sS,
+                                                double2* __restrict__ W, long sWl, Dims d, const double* __restrict__ T0t,
                                                 const double* __restrict__ T2t, const double* __restrict__ T4q,
-                                                const double* __restrict__ T2q) {
+                                                const double* __restrict__ T2q, long sQ, const int* __restrict__ ids) {
+    {
+        const long slot = ids[blockIdx.y];
+        sv += slot * sS;
+        se += slot * sS;
+        W += slot * sWl;
+        T4q += slot * sQ;
+        T2q += slot * sQ;
+    }
     static_assert(NPB % kTvCols == 0, "pencil tile");
     extern __shared__ __align__(16) unsigned char tv_lds_raw[];
     constexpr int LDX = NPB + 1;  // padded row: the transposing stores hit different banks
@@ -340,7 +358,8 @@ __global__ __launch_bounds__(384) void k_lin_tv(const double2* __restrict__ sv, 
 // The same operator products on the matrix cores: W[m] = T_m X with X the nz x 2P real
 // matrix of (Re, Im) columns, by v_mfma_f64_16x16x4_f64 (A: lane l holds A[l%16][l/16],
 // B: lane l holds B[l/16][l%16], D: item r of lane l is D[l/16 + 4r][l%16]).
-// Block: operator m = blockIdx.y, 32 pencils, 4 waves of 8 pencils (16 columns).  A wave
+// Block: operator m = blockIdx.y % 5 of member blockIdx.y / 5, 32 pencils, 4 waves of 8
+// pencils (16 columns).  A wave
 // keeps its B fragments (S = ceil(nz/4) k-steps) in registers; the A fragments of
 // kTvChunk row tiles are staged in LDS once per block and chunk.  Tf[m][rt][s][64]: the
 // 16x4 tile (rows 16rt.., columns 4s..) of T_m in A-operand lane order, zero outside
@@ -352,10 +371,18 @@ constexpr int kTvBlockPencils = 32;
 
 template <int S>
 __global__ __launch_bounds__(256) void k_lin_tv_mfma(const double2* __restrict__ sv, const double2* __restrict__ se,
-                                                     double2* __restrict__ W, Dims d, const double* __restrict__ Tf,
-                                                     int RT) {
+                                                     long sS, double2* __restrict__ W, long sWl, Dims d,
+                                                     const double* __restrict__ Tf, long sTf, int RT,
+                                                     const int* __restrict__ ids) {
     extern __shared__ __align__(16) double tvA[];  // [kTvChunk][S][64]
-    const int m = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int m = blockIdx.y % 5, tid = threadIdx.x, lane = tid & 63;
+    {
+        const long slot = ids[blockIdx.y / 5];
+        sv += slot * sS;
+        se += slot * sS;
+        W += slot * sWl;
+        Tf += slot * sTf;
+    }
     const int n = d.nz, col = lane & 15, kq = lane >> 4;
     const int p = blockIdx.x * kTvBlockPencils + (tid >> 6) * 8 + (col >> 1);
     const bool okp = p < d.P;
@@ -432,16 +459,47 @@ __device__ __forceinline__ void gemv_rows2(const double2* __restrict__ M, const 
 // source/rhs.cpp:205-215), v solve (solveEq, source/solver.cpp:107-123), T0 v^{n+1} (one
 // pass over T0), eta right-hand side (_get_linear_omega_y, source/linear.cpp:127-171), eta
 // solve.  Block 0 runs the mean flow of slot (0,0) (_get_linear_u0/_w0,
-// source/linear.cpp:173-205) and the constant-flow-rate correction.  Thread r = row r.
+// source/linear.cpp:173-205) and the constant-flow-rate correction.  Thread r = row r;
+// matrix slot blockIdx.x, member blockIdx.y.
 //   v rows r = 4..N, p = r-2:  (T4q v + k2 T2 v - k0 T0 v) + i (a2 T2 v + a0 T0 v)
 //   eta rows r = 2..N, p = r-1: (T2q eta + c0 T0 eta) + i ci T0 eta + i e T0 (v^n + v^{n+1})
 // -------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lin_solve(LinState st, const double2* __restrict__ W, Dims d, LinParams lp,
+__global__ __launch_bounds__(256) void k_lin_solve(double2* __restrict__ SV, double2* __restrict__ SE, long sS,
+                                                   const double2* __restrict__ NL, long sNL,
+                                                   const double2* __restrict__ MINV, long sM,
+                                                   const double2* __restrict__ W, long sWl, Dims d, Members mb,
                                                    const double* __restrict__ T0t, const double* __restrict__ U,
-                                                   const double* __restrict__ dU, const double* __restrict__ ddU,
+                                                   const double* __restrict__ dU, const double* __restrict__ ddU, long sU,
                                                    const double* __restrict__ qw) {
     __shared__ double2 rhs[2][256], vnew[2][256];
     __shared__ double2 s_f;
+    LinState st;
+    LinParams lp;
+    {
+        const int ms = mb.ids[blockIdx.y];
+        const MemberParams& mp = mb.mp[ms];
+        const int ph = mb.phase[ms];
+        const long PS = (long)d.P * d.nz;
+        const double2* h = NL + ms * sNL;
+        st.nlv = h + ph * PS;
+        st.nlvp = h + (1 - ph) * PS;
+        st.nle = h + (2 + ph) * PS;
+        st.nlep = h + (3 - ph) * PS;
+        st.sv = SV + ms * sS;
+        st.se = SE + ms * sS;
+        st.minv = MINV + ms * sM;
+        st.qw_target_re = mp.q_re;
+        st.qw_target_im = mp.q_im;
+        st.flow_control = mp.flow_control;
+        lp.aphi = mp.aphi;
+        lp.beta = mp.beta;
+        lp.Re = mp.Re;
+        lp.dt = mp.dt;
+        W += ms * sWl;
+        U += ms * sU;
+        dU += ms * sU;
+        ddU += ms * sU;
+    }
     const int slot = blockIdx.x;
     const int kyh = slot / d.hnx, kx = slot - kyh * d.hnx;
     const int n = d.nz, r = threadIdx.x;
@@ -564,4 +622,10 @@ __global__ __launch_bounds__(256) void k_lin_solve(LinState st, const double2* _
             st.sv[po1 + r] = vnew[1][r];
         }
     }
+}
+
+// AB2 history swap of every listed member (source/rhs.cpp:107-108): the buffer that held
+// N^{n-1} becomes the current one and receives the new N.
+__global__ void k_flip(int* __restrict__ phase, const int* __restrict__ ids, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) phase[ids[i]] ^= 1;
 }

@@ -58,6 +58,7 @@ __device__ __forceinline__ void staged(int total, int tid, int nthr, Load load, 
 //        d_0 = a_0, d_k = a_k/2 e^{-i pi k/(2N)}, d_{2N-k} = a_k/2 e^{+i pi k/(2N)} (0<k<N),
 //        d_N = 0 -- two length-2N FFTs in a 2N-point LDS tile instead of one of length 4N.
 // tw4: exp(-2 pi i k/(4N)), k < 4N.  Tile: TB pencils (kx fixed); output Y[f][kx][q][ky].
+// blockIdx.z = member * 6 + field; per-slot arrays at F + slot*sF, Y + slot*sY.
 // -------------------------------------------------------------------------------------
 __device__ __forceinline__ void s2p_even_ext(double2* s, double2 a, int k, int N) {
     const int L = 2 * N;
@@ -83,11 +84,15 @@ __device__ __forceinline__ void s2p_odd_ext(double2* s, double2 a, int k, int N,
 }
 
 template <int PAD, int FN = 0>  // FN: compile-time 2N or 0
-__global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F, long fstride, double2* __restrict__ Y,
-                                                  long ystride, Dims d, FftPlan pl, const double2* __restrict__ tw4,
-                                                  int TB, int ld) {
+__global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F, long fstride, long sF,
+                                                  double2* __restrict__ Y, long ystride, long sY, Dims d, FftPlan pl,
+                                                  const double2* __restrict__ tw4, int TB, int ld,
+                                                  const int* __restrict__ ids) {
     RPCF_LDS(double2);
-    const int f = blockIdx.z, kx = blockIdx.y, ky0 = blockIdx.x * TB;
+    const int f = blockIdx.z % 6, kx = blockIdx.y, ky0 = blockIdx.x * TB;
+    const long slot = ids[blockIdx.z / 6];
+    F += slot * sF;
+    Y += slot * sY;
     const int nb = min(TB, d.ny - ky0);
     const int N = FN ? FN / 2 : d.N, nz = N + 1, nout = PAD ? 2 * N + 1 : nz;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -148,14 +153,19 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
 // -------------------------------------------------------------------------------------
 // Span inverse FFT with the 3/2 zero padding: the span C2C half of the reference's 2D
 // Z2D (source/transform.cu:176) after transpose_backward + setZeros
-// (source/transpose.cu:358-381, source/transform.cu:355-386).  Tile: TK kx columns of one
+// (source/transpose.cu:358-381, source/transform.cu:355-386).  blockIdx.z = member * 6 +
+// field.  Tile: TK kx columns of one
 // plane.  Y[f][kx][q][ky] -> W[f][q][s][kx] for kx < nxh (the x-Nyquist column is zero).
 // -------------------------------------------------------------------------------------
 template <int FN = 0>
-__global__ __launch_bounds__(256) void k_span_inv(const double2* __restrict__ Y, long ystride, double2* __restrict__ W, long wstride, Dims d,
-                           int planes, FftPlan pl, int TK, int ld) {
+__global__ __launch_bounds__(256) void k_span_inv(const double2* __restrict__ Y, long ystride, long sY, double2* __restrict__ W,
+                                                  long wstride, long sW, Dims d, int planes, FftPlan pl, int TK, int ld,
+                                                  const int* __restrict__ ids) {
     RPCF_LDS(double2);
-    const int f = blockIdx.z, q = blockIdx.y, kx0 = blockIdx.x * TK;
+    const int f = blockIdx.z % 6, q = blockIdx.y, kx0 = blockIdx.x * TK;
+    const long slot = ids[blockIdx.z / 6];
+    Y += slot * sY;
+    W += slot * sW;
     const int nk = min(TK, d.nxh - kx0);
     const int my = FN ? FN : d.my, tid = threadIdx.x, nthr = blockDim.x;
     const double2* in = Y + f * ystride;
@@ -213,15 +223,22 @@ __device__ __forceinline__ void r2c_unpack(const double2* z, int k, int mx, doub
 // Fused physical-space stage: x C2R of the six fields, Lamb vector + Coriolis term
 // (computeLambDevice, source/nonlinear.cu:409-428, base flow at padded point q as in
 // source/nonlinear.cu:400-403), x R2C of the three Lamb components and the
-// 1/(mx my) normalisation of source/transform.cu:267.  One block: one plane q, RP
+// 1/(mx my) normalisation of source/transform.cu:267.  Member blockIdx.z (its Ro and
+// padded base profile meanU).  One block: one plane q, RP
 // pairs of span rows.  W[6][pz][my][nxh] -> Z2[3][pz][my][hnx].
 // -------------------------------------------------------------------------------------
 template <int FN = 0>
-__global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W, long wstride, double2* __restrict__ Z2, long zstride,
-                            int zld, const double* __restrict__ meanU, double Ro, double fac, Dims d, FftPlan pl,
-                            int RP, int ld) {
+__global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W, long wstride, long sW,
+                                                   double2* __restrict__ Z2, long zstride, long sZ, int zld,
+                                                   const double* __restrict__ meanU, long sPz, double fac, Dims d,
+                                                   FftPlan pl, int RP, int ld, Members mb) {
     RPCF_LDS(double2);
     const int q = blockIdx.y, rp0 = blockIdx.x * RP;
+    const int slot = mb.ids[blockIdx.z];
+    const double Ro = mb.mp[slot].Ro;
+    W += slot * sW;
+    Z2 += slot * sZ;
+    meanU += slot * sPz;
     const int nrp = min(RP, d.my / 2 - rp0);
     const int mx = FN ? FN : d.mx, my = d.my, nxh = d.nxh, tid = threadIdx.x, nthr = blockDim.x;
     // packed spectra Z = X_A + i X_B of the row pairs (c2r_pack, element by element): each
@@ -341,13 +358,18 @@ __global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, 
 // -------------------------------------------------------------------------------------
 // Span forward FFT + truncation to the ny kept modes: the span C2C half of the
 // reference's D2Z (source/transform.cu:240) and transpose_forward
-// (source/transpose.cu:223-257).  Z2[g][q][s][kx] -> Z1[g][kx][q][ky].
+// (source/transpose.cu:223-257).  Z2[g][q][s][kx] -> Z1[g][kx][q][ky]; blockIdx.z =
+// member * 3 + component.
 // -------------------------------------------------------------------------------------
 template <int FN = 0>
-__global__ __launch_bounds__(256) void k_span_fwd(const double2* __restrict__ Z2, long zstride, int zld, double2* __restrict__ Z1,
-                           long ostride, Dims d, int planes, FftPlan pl, int TK, int ld) {
+__global__ __launch_bounds__(256) void k_span_fwd(const double2* __restrict__ Z2, long zstride, long sZ, int zld,
+                                                  double2* __restrict__ Z1, long ostride, long sY, Dims d, int planes,
+                                                  FftPlan pl, int TK, int ld, const int* __restrict__ ids) {
     RPCF_LDS(double2);
-    const int g = blockIdx.z, q = blockIdx.y, kx0 = blockIdx.x * TK;
+    const int g = blockIdx.z % 3, q = blockIdx.y, kx0 = blockIdx.x * TK;
+    const long slot = ids[blockIdx.z / 3];
+    Z2 += slot * sZ;
+    Z1 += slot * sY;
     const int nk = min(TK, d.hnx - kx0);
     const int my = FN ? FN : d.my, ny = d.ny, tid = threadIdx.x, nthr = blockDim.x;
     const double2* in = Z2 + g * zstride;
@@ -463,9 +485,9 @@ __device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int
 // -------------------------------------------------------------------------------------
 // Forward Chebyshev transform of the three Lamb components, rhsNonlinearKernel
 // (source/nonlinear.cu:146-242) and the unpadded s2p of N_v and N_eta
-// (source/rhs.cpp:96-97).  Tile: TB <= kChebTile pencils (kx fixed), 256 threads.
-// Z1[3][hnx][pz][ny] -> NLv, NLe [P][nz] (values at the nz collocation points; pencil
-// (0,0) keeps L_x, L_y).
+// (source/rhs.cpp:96-97).  Tile: TB <= kChebTile pencils (kx fixed), 256 threads; member
+// blockIdx.z.  Z1[3][hnx][pz][ny] -> the current N_v, N_eta [P][nz] of the slot's history
+// NL[4][P][nz] (values at the nz collocation points; pencil (0,0) keeps L_x, L_y).
 //
 // The reference's p2s is a length-4N FFT of the even extension u of the pz values
 // (cheby_pre_p2s/cheby_post_p2s, source/transform.cu:494-536).  Split by the parity of
@@ -480,11 +502,17 @@ __device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int
 // as [TB][2][nz]) and 256 chunk totals.
 // -------------------------------------------------------------------------------------
 template <int FN = 0>  // FN: compile-time 2N or 0
-__global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, long zstride, double2* __restrict__ NLv,
-                                                double2* __restrict__ NLe, Dims d, FftPlan plN,
-                                                const double2* __restrict__ tw4, double aphi, double beta, int TB,
-                                                int ld) {
+__global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, long zstride, long sY,
+                                                double2* __restrict__ NL, long sNL, Dims d, FftPlan plN,
+                                                const double2* __restrict__ tw4, int TB, int ld, Members mb) {
     RPCF_LDS(double2);
+    const int mslot = mb.ids[blockIdx.z];
+    const int ph = mb.phase[mslot];
+    const double aphi = mb.mp[mslot].aphi, beta = mb.mp[mslot].beta;
+    const long PS = (long)d.P * d.nz;
+    Z1 += mslot * sY;
+    double2* NLv = NL + mslot * sNL + ph * PS;        // N_v, current
+    double2* NLe = NL + mslot * sNL + (2 + ph) * PS;  // N_eta, current
     constexpr int KC = FN ? (kChebTile * (FN / 2 + 1) + 255) / 256 : (kChebTile * 257 + 255) / 256;
     constexpr int CHM = FN ? (FN / 4 + 1 + 7) / 8 : (128 + 1 + 7) / 8;  // chunk length bound
     const int kx = blockIdx.y, ky0 = blockIdx.x * TB;

@@ -23,3 +23,20 @@ __host__ __device__ __forceinline__ int ky_of_span(const Dims& d, int s) {
     if (s > d.my - d.ny / 2) return s - (d.my - d.ny);
     return -1;
 }
+
+// Scalars of one simulation (slot) of a group: wavenumber scales aphi = Lx/2, beta = Lz/2
+// (include/data.h:321-322), Re, dt, rotation number, the flow-rate target (bulk of u0 at
+// set_state) and the flow-control switch.
+struct MemberParams {
+    double aphi, beta, Re, dt, Ro;
+    double q_re, q_im;
+    int flow_control;
+};
+
+// Member addressing of one grouped launch: launch member j runs slot ids[j]; a per-slot
+// array of slot s starts at its base pointer + s * (its slot stride, in elements).
+struct Members {
+    const int* ids;
+    const MemberParams* mp;
+    const int* phase;  // per slot: N history buffer "current" = phase, "previous" = 1 - phase
+};

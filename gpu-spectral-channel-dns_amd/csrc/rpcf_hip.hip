@@ -1,7 +1,12 @@
-// rpcf_hip.hip -- MI355X (gfx950) implementation of the rpcf C ABI (include/rpcf.h):
-// context, device memory, launches, and the host side of the reference's step
-// orchestration (nextStep, source/solver.cpp:35-101), with the whole step resident on
-// the GPU (no per-step host<->device copies, no helper thread).
+// rpcf_hip.hip -- MI355X (gfx950) implementation of the rpcf C ABI (include/rpcf.h,
+// include/rpcf_group.h): context, device memory, launches, and the host side of the
+// reference's step orchestration (nextStep, source/solver.cpp:35-101), with the whole step
+// resident on the GPU (no per-step host<->device copies, no helper thread).
+//
+// A context holds `capacity` simulations ("slots") of one grid shape.  Every per-slot
+// array is one allocation [capacity][...]; a launch covers a list of slots (its member
+// coordinate j runs slot ids[j]), so one kernel launch advances all listed simulations.
+// The single-simulation ABI of rpcf.h is slot 0 of a context of capacity 1.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,6 +17,7 @@
 #include <vector>
 
 #include "../../include/rpcf.h"
+#include "../../include/rpcf_group.h"
 #include "host_prep.hpp"
 #include "k_linear.hpp"
 #include "k_transform.hpp"
@@ -25,53 +31,73 @@ const char* const kKernelNames[KID_COUNT] = {"k_velocity", "k_cheb_s2p", "k_span
                                              "k_span_fwd", "k_p2s_nl",   "k_lin_tv",   "k_lin_solve"};
 
 constexpr int kFftThreads = 256;
+constexpr int kTvPencils = 16;  // pencils per block of the fallback k_lin_tv
 
 struct Pending {
     int kid;
     hipEvent_t a, b;
 };
 
-}  // namespace
-
+// the six spectral fields F of a slot relative to its state: STALE = velocity(state), not
+// computed yet; FRESH = computed from the current state; EXPLICIT = set by the caller, or
+// held over a linear step, so not derivable from the state
 enum { FIELDS_STALE = 0, FIELDS_FRESH = 1, FIELDS_EXPLICIT = 2 };
 
-struct rpcf_ctx {
+struct SlotHost {
+    int configured = 0;
+    int have_state = 0;
+    int fields = FIELDS_STALE;
+    int phase = 0;  // mirror of the device history phase
     rpcf_params prm{};
+    MemberParams mp{};
+};
+
+}  // namespace
+
+struct rpcf_ctx {
+    rpcf_params shape{};
     Dims d{};
-    LinParams lp{};
-    double Ro = 0.0, fac = 0.0;
+    int capacity = 1;
+    double fac = 0.0;
     int device = 0;
     hipStream_t stream = nullptr;
     int lds_limit = 65536;
     // tiles: sequences per block and LDS row stride (elements) of each transform kernel
     int TB_s2p = 1, TB_nl = 1, TK_inv = 1, TK_fwd = 1, RP = 1;
-    int ld_zn = 0, ld_my = 0, ld_mx = 0;
+    int ld_zn = 0, ld_my = 0, ld_mx = 0, zld = 0;
     FftPlan pl_x{}, pl_y{}, pl_zp{}, pl_zn{};
     int cap_x = 0, cap_y = 0, cap_zp = 0, cap_zn = 0;  // fft_capacity of each plan
-    // device memory
-    double2* F = nullptr;   // [6][P][nz]
-    double2* sv = nullptr;  // [P][nz] v-hat (u0 in pencil 0)
-    double2* se = nullptr;  // [P][nz] eta-hat (w0 in pencil 0)
-    double2* nl[4] = {nullptr, nullptr, nullptr, nullptr};  // N_v, N_v prev, N_eta, N_eta prev
-    double2* Y = nullptr;   // [6][hnx][pz][ny] (also Z1 [3][hnx][pz][ny])
-    double2* W = nullptr;   // [6][pz][my][nxh]
-    double2* Z2 = nullptr;  // [3][pz][my][zld], zld = hnx rounded up to 8 (128-byte rows)
-    int zld = 0;
-    double2* minv = nullptr;
-    double2* Wlin = nullptr;  // [5][P][nz] operator products of the linear right-hand sides
-    double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *U = nullptr, *dU = nullptr, *ddU = nullptr;
-    // transposed tables for k_linear: T0, T2, and the quotients T4*dt*0.5/Re, T2*0.5/Re*dt
-    double *T0t = nullptr, *T2t = nullptr, *T4q = nullptr, *T2q = nullptr;
-    double* Tf = nullptr;  // [5][RT][S][64] A-operand tiles of k_lin_tv_mfma
-    int tv_rt = 0, tv_s = 0;
-    double *meanU = nullptr, *qw = nullptr, *tw = nullptr;
-    std::vector<double> qw_host;
-    double2 q_target = make_double2(0.0, 0.0);
-    int have_state = 0;
-    // the six spectral fields F relative to the state: FIELDS_STALE = velocity(state), not
-    // computed yet; FIELDS_FRESH = computed from the current state; FIELDS_EXPLICIT = set by
-    // the caller, or held over a linear step, so not derivable from the state
-    int fields = 0;
+    int tv_rt = 0, tv_s = 0;                           // MFMA row tiles and k-steps of the operator products
+    // per-slot arrays ([capacity][...]) and their slot strides in elements
+    double2* F = nullptr;     // [6][P][nz] u, v, w, ox, oy, oz
+    double2* SV = nullptr;    // [P][nz] v-hat (u0 in pencil 0)
+    double2* SE = nullptr;    // [P][nz] eta-hat (w0 in pencil 0)
+    double2* NL = nullptr;    // [4][P][nz] N_v (2 buffers), N_eta (2 buffers); current = phase
+    double2* Y = nullptr;     // [6][hnx][pz][ny] (also Z1 [3][hnx][pz][ny])
+    double2* W = nullptr;     // [6][pz][my][nxh]
+    double2* Z2 = nullptr;    // [3][pz][my][zld], zld = hnx rounded up to 8 (128-byte rows)
+    double2* WL = nullptr;    // [5][P][nz] operator products of the linear right-hand sides
+    double2* MINV = nullptr;  // [2][nslot][nz*nz] inverses, column-major
+    double* TF = nullptr;     // [5][RT][S][64] MFMA A-operand tiles of T0, T2, T4q, T0, T2q
+    double* T4Q = nullptr;    // [nz*nz] T4*dt*0.5/Re, transposed (fallback operator products)
+    double* T2Q = nullptr;    // [nz*nz] T2*0.5/Re*dt, transposed
+    double *U = nullptr, *DU = nullptr, *DDU = nullptr;  // [nz] base profile at the collocation points
+    double* MEANU = nullptr;                             // [pz] base profile at the padded points
+    MemberParams* MP = nullptr;                          // [capacity]
+    int* PH = nullptr;                                   // [capacity] history phase
+    long sF = 0, sS = 0, sNL = 0, sY = 0, sW = 0, sZ = 0, sWl = 0, sM = 0, sTf = 0, sQ = 0, sU = 0, sPz = 0;
+    // shape tables shared by all slots
+    double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *T0t = nullptr, *T2t = nullptr, *qw = nullptr, *tw = nullptr;
+    std::vector<double> T0h, T2h, T4h, qw_host;
+    // slot-id lists on the device: ids_zero = {0}; ids = the list of the last grouped launch
+    int* ids_dev = nullptr;
+    int* ids_zero = nullptr;
+    std::vector<int> ids_cur;
+    std::vector<SlotHost> slots;
+    // graph of one grouped step of g_ids
+    int use_graphs = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<int> g_ids;
     // per-kernel timing with hipEvents on the context stream
     int timing = 0;
     std::vector<Pending> pending;
@@ -120,6 +146,34 @@ int check_sizes(const rpcf_params* prm, Dims* out) {
     return RPCF_OK;
 }
 
+// per-slot element counts of every per-slot array (= the slot strides)
+struct SlotSizes {
+    long F, S, NL, Y, W, Z, Wl, M, Tf, Q, U, Pz;
+    size_t bytes() const {
+        return sizeof(double2) * (size_t)(F + 2 * S + NL + Y + W + Z + Wl + M) +
+               sizeof(double) * (size_t)(Tf + 2 * Q + 3 * U + Pz) + sizeof(MemberParams) + 2 * sizeof(int);
+    }
+};
+
+SlotSizes slot_sizes(const Dims& d) {
+    SlotSizes s;
+    const long P = d.P, nz = d.nz, zld = (d.hnx + 7) & ~7;
+    const long RT = (nz + 15) / 16, S = (nz + 3) / 4;
+    s.F = 6 * P * nz;
+    s.S = P * nz;
+    s.NL = 4 * P * nz;
+    s.Y = 6L * d.hnx * d.pz * d.ny;
+    s.W = 6L * d.pz * d.my * d.nxh;
+    s.Z = 3L * d.pz * d.my * zld;
+    s.Wl = 5 * P * nz;
+    s.M = 2L * d.nslot * nz * nz;
+    s.Tf = 5 * RT * S * 64;
+    s.Q = nz * nz;
+    s.U = nz;
+    s.Pz = d.pz;
+    return s;
+}
+
 hipEvent_t get_event(rpcf_ctx* c) {
     if (!c->pool.empty()) {
         hipEvent_t e = c->pool.back();
@@ -147,17 +201,19 @@ int sync_ctx(rpcf_ctx* c) {
     return RPCF_OK;
 }
 
+// Launch with optional HIP-event timing (never while a graph is being captured).
 template <typename F>
-int launch(rpcf_ctx* c, int kid, F&& fn) {
+int launch(rpcf_ctx* c, int kid, F&& fn, bool capturing = false) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (c->timing) {
+    const bool timed = c->timing && !capturing;
+    if (timed) {
         a = get_event(c);
         (void)hipEventRecord(a, c->stream);
     }
     fn();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, RPCF_EDEVICE, "launch %s: %s", kKernelNames[kid], hipGetErrorString(e));
-    if (c->timing) {
+    if (timed) {
         b = get_event(c);
         (void)hipEventRecord(b, c->stream);
         c->pending.push_back({kid, a, b});
@@ -225,142 +281,202 @@ int download_spec(rpcf_ctx* c, double* dst, const double2* src, long stride) {
     return RPCF_OK;
 }
 
-// ---- step pieces (asynchronous on c->stream) ------------------------------------------
-int enq_velocity(rpcf_ctx* c) {
-    const Dims d = c->d;
-    const int thr = 64 * ((d.nz + 63) / 64);
-    return launch(c, KID_VELOCITY, [&] {
-        k_velocity<<<d.P, thr, 0, c->stream>>>(c->sv, c->se, c->F, (long)d.P * d.nz, d, c->lp);
-    });
+// history buffer of a slot: which = 0 N_v, 1 N_eta; prev selects N^{n-1}
+double2* hist(rpcf_ctx* c, int slot, int which, int prev) {
+    const int ph = c->slots[slot].phase;
+    const int b = (prev ? 1 - ph : ph) + 2 * which;
+    return c->NL + slot * c->sNL + (long)b * c->sS;
 }
 
-int enq_nonlinear(rpcf_ctx* c) {
+int check_slot(rpcf_ctx* c, int slot) {
+    if (!c) return RPCF_EINVAL;
+    if (slot < 0 || slot >= c->capacity) return fail(c, RPCF_EINVAL, "slot %d outside 0..%d", slot, c->capacity - 1);
+    if (!c->slots[slot].configured) return fail(c, RPCF_ESTATE, "slot %d has no member", slot);
+    return RPCF_OK;
+}
+
+// Make `ids` the device slot list of the following grouped launches.
+int set_ids(rpcf_ctx* c, const std::vector<int>& ids) {
+    if (ids == c->ids_cur) return RPCF_OK;
+    HIPCHK(c, hipMemcpyAsync(c->ids_dev, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ids_cur = ids;
+    return RPCF_OK;
+}
+
+Members members(rpcf_ctx* c, const int* ids) {
+    Members m;
+    m.ids = ids;
+    m.mp = c->MP;
+    m.phase = c->PH;
+    return m;
+}
+
+// ---- step pieces over a device slot list (asynchronous on c->stream) ---------------------
+int enq_velocity(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
+    const Dims d = c->d;
+    const int thr = 64 * ((d.nz + 63) / 64);
+    return launch(
+        c, KID_VELOCITY,
+        [&] {
+            k_velocity<<<dim3(d.P, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->F, (long)d.P * d.nz, c->sF, d,
+                                                            members(c, ids));
+        },
+        cap);
+}
+
+// getNonlinear for the listed slots; with_velocity: form their fields from the state first
+int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool cap = false) {
     const Dims d = c->d;
     int rc;
     const long fS = (long)d.P * d.nz, yS = (long)d.hnx * d.pz * d.ny, wS = (long)d.pz * d.my * d.nxh,
                zS = (long)d.pz * d.my * c->zld;
     const double2* tw4 = (const double2*)c->pl_zp.tw;
-    if (c->fields == FIELDS_STALE) {
-        if ((rc = enq_velocity(c))) return rc;
-        c->fields = FIELDS_FRESH;
-    }
+    const Members mb = members(c, ids);
+    if (with_velocity && (rc = enq_velocity(c, ids, n, cap))) return rc;
     {
         const int TB = c->TB_s2p, ld = c->ld_zn;
-        dim3 g((d.ny + TB - 1) / TB, d.hnx, 6);
+        dim3 g((d.ny + TB - 1) / TB, d.hnx, 6 * n);
         size_t lds = sizeof(double2) * TB * ld;
-        rc = launch(c, KID_S2P, [&] {
-            switch (2 * d.N) {
-#define RPCF_CASE(L) case L: k_cheb_s2p<1, L><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->Y, yS, d, c->pl_zn, tw4, TB, ld); break;
-                RPCF_FFT_LENGTHS(RPCF_CASE)
+        rc = launch(
+            c, KID_S2P,
+            [&] {
+                switch (2 * d.N) {
+#define RPCF_CASE(L) case L: k_cheb_s2p<1, L><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, c->Y, yS, c->sY, d, c->pl_zn, tw4, TB, ld, ids); break;
+                    RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-                default: k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->Y, yS, d, c->pl_zn, tw4, TB, ld);
-            }
-        });
+                    default:
+                        k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, c->Y, yS, c->sY, d, c->pl_zn,
+                                                                          tw4, TB, ld, ids);
+                }
+            },
+            cap);
         if (rc) return rc;
     }
     {
         const int TK = c->TK_inv, ld = c->ld_my;
-        dim3 g((d.nxh + TK - 1) / TK, d.pz, 6);
+        dim3 g((d.nxh + TK - 1) / TK, d.pz, 6 * n);
         size_t lds = sizeof(double2) * TK * ld;
-        rc = launch(c, KID_SPAN_INV, [&] {
-            switch (d.my) {
-#define RPCF_CASE(L) case L: k_span_inv<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->W, wS, d, d.pz, c->pl_y, TK, ld); break;
-                RPCF_FFT_LENGTHS(RPCF_CASE)
+        rc = launch(
+            c, KID_SPAN_INV,
+            [&] {
+                switch (d.my) {
+#define RPCF_CASE(L) case L: k_span_inv<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->W, wS, c->sW, d, d.pz, c->pl_y, TK, ld, ids); break;
+                    RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-                default: k_span_inv<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->W, wS, d, d.pz, c->pl_y, TK, ld);
-            }
-        });
+                    default:
+                        k_span_inv<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->W, wS, c->sW, d, d.pz,
+                                                                       c->pl_y, TK, ld, ids);
+                }
+            },
+            cap);
         if (rc) return rc;
     }
     {
         const int RP = c->RP, ld = c->ld_mx;
-        dim3 g((d.my / 2 + RP - 1) / RP, d.pz);
+        dim3 g((d.my / 2 + RP - 1) / RP, d.pz, n);
         size_t lds = sizeof(double2) * 6 * RP * ld;
-        rc = launch(c, KID_LAMB, [&] {
-            switch (d.mx) {
-#define RPCF_CASE(L) case L: k_lamb_phys<L><<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->zld, c->meanU, c->Ro, c->fac, d, c->pl_x, RP, ld); break;
-                RPCF_FFT_LENGTHS(RPCF_CASE)
+        rc = launch(
+            c, KID_LAMB,
+            [&] {
+                switch (d.mx) {
+#define RPCF_CASE(L) case L: k_lamb_phys<L><<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->sW, c->Z2, zS, c->sZ, c->zld, c->MEANU, c->sPz, c->fac, d, c->pl_x, RP, ld, mb); break;
+                    RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-                default: k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->Z2, zS, c->zld, c->meanU, c->Ro, c->fac, d,
-                                                                         c->pl_x, RP, ld);
-            }
-        });
+                    default:
+                        k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->sW, c->Z2, zS, c->sZ, c->zld,
+                                                                        c->MEANU, c->sPz, c->fac, d, c->pl_x, RP, ld,
+                                                                        mb);
+                }
+            },
+            cap);
         if (rc) return rc;
     }
     {
         const int TK = c->TK_fwd, ld = c->ld_my;
-        dim3 g((d.hnx + TK - 1) / TK, d.pz, 3);
+        dim3 g((d.hnx + TK - 1) / TK, d.pz, 3 * n);
         size_t lds = sizeof(double2) * TK * ld;
-        rc = launch(c, KID_SPAN_FWD, [&] {
-            switch (d.my) {
-#define RPCF_CASE(L) case L: k_span_fwd<L><<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->zld, c->Y, yS, d, d.pz, c->pl_y, TK, ld); break;
-                RPCF_FFT_LENGTHS(RPCF_CASE)
+        rc = launch(
+            c, KID_SPAN_FWD,
+            [&] {
+                switch (d.my) {
+#define RPCF_CASE(L) case L: k_span_fwd<L><<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->sZ, c->zld, c->Y, yS, c->sY, d, d.pz, c->pl_y, TK, ld, ids); break;
+                    RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-                default: k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->zld, c->Y, yS, d, d.pz, c->pl_y, TK, ld);
-            }
-        });
+                    default:
+                        k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->sZ, c->zld, c->Y, yS, c->sY, d,
+                                                                       d.pz, c->pl_y, TK, ld, ids);
+                }
+            },
+            cap);
         if (rc) return rc;
     }
     // AB2 history swap (source/rhs.cpp:107-108): the new N lands in the old "previous" buffers
-    std::swap(c->nl[0], c->nl[1]);
-    std::swap(c->nl[2], c->nl[3]);
+    k_flip<<<1, 64, 0, c->stream>>>(c->PH, ids, n);
     {
         const int TB = c->TB_nl, ld = c->ld_zn;
-        dim3 g((d.ny + TB - 1) / TB, d.hnx);
+        dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
         size_t lds = sizeof(double2) * ((size_t)TB * ld + 256);
-        rc = launch(c, KID_P2S_NL, [&] {
-            switch (2 * d.N) {
-#define RPCF_CASE(L) case L: k_p2s_nl<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->nl[0], c->nl[2], d, c->pl_zn, tw4, c->lp.aphi, c->lp.beta, TB, ld); break;
-                RPCF_FFT_LENGTHS(RPCF_CASE)
+        rc = launch(
+            c, KID_P2S_NL,
+            [&] {
+                switch (2 * d.N) {
+#define RPCF_CASE(L) case L: k_p2s_nl<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->NL, c->sNL, d, c->pl_zn, tw4, TB, ld, mb); break;
+                    RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-                default: k_p2s_nl<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->nl[0], c->nl[2], d, c->pl_zn, tw4,
-                                                                      c->lp.aphi, c->lp.beta, TB, ld);
-            }
-        });
+                    default:
+                        k_p2s_nl<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->NL, c->sNL, d, c->pl_zn, tw4,
+                                                                     TB, ld, mb);
+                }
+            },
+            cap);
     }
     return rc;
 }
 
-constexpr int kTvPencils = 16;  // pencils per block of k_lin_tv
-
-int enq_linear(rpcf_ctx* c) {
+int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
     const Dims d = c->d;
-    LinState st;
-    st.nlv = c->nl[0]; st.nlvp = c->nl[1]; st.nle = c->nl[2]; st.nlep = c->nl[3];
-    st.sv = c->sv; st.se = c->se; st.minv = c->minv;
-    st.qw_target_re = c->q_target.x; st.qw_target_im = c->q_target.y;
-    st.flow_control = c->prm.flow_control;
-    int rc = launch(c, KID_LIN_TV, [&] {
-        dim3 g((d.P + kTvBlockPencils - 1) / kTvBlockPencils, 5);
-        const size_t lds = sizeof(double) * kTvChunk * c->tv_s * 64;
-        switch (2 * d.N) {
-#define RPCF_CASE(L)                                                                                    \
-    case L:                                                                                             \
-        k_lin_tv_mfma<(L / 2 + 4) / 4><<<g, 256, lds, c->stream>>>(c->sv, c->se, c->Wlin, d, c->Tf, c->tv_rt); \
+    int rc = launch(
+        c, KID_LIN_TV,
+        [&] {
+            const size_t lds = sizeof(double) * kTvChunk * c->tv_s * 64;
+            dim3 g((d.P + kTvBlockPencils - 1) / kTvBlockPencils, 5 * n);
+            switch (2 * d.N) {
+#define RPCF_CASE(L)                                                                                             \
+    case L:                                                                                                      \
+        k_lin_tv_mfma<(L / 2 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, \
+                                                                   c->sTf, c->tv_rt, ids);                      \
         break;
-            RPCF_FFT_LENGTHS(RPCF_CASE)
+                RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-            default: {
-                const int gv = (d.P + kTvPencils - 1) / kTvPencils;
-                const size_t lv = sizeof(double2) * 2 * (size_t)d.nz * (kTvPencils + 1);
-                k_lin_tv<kTvPencils><<<gv, 384, lv, c->stream>>>(c->sv, c->se, c->Wlin, d, c->T0t, c->T2t, c->T4q,
-                                                                 c->T2q);
+                default: {
+                    dim3 gv((d.P + kTvPencils - 1) / kTvPencils, n);
+                    const size_t lv = sizeof(double2) * 2 * (size_t)d.nz * (kTvPencils + 1);
+                    k_lin_tv<kTvPencils><<<gv, 384, lv, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->T0t,
+                                                                     c->T2t, c->T4Q, c->T2Q, c->sQ, ids);
+                }
             }
-        }
-    });
+        },
+        cap);
     if (rc) return rc;
-    return launch(c, KID_LINEAR, [&] {
-        const int thr = 64 * ((d.nz + 63) / 64);
-        k_lin_solve<<<d.nslot, thr, 0, c->stream>>>(st, c->Wlin, d, c->lp, c->T0t, c->U, c->dU, c->ddU, c->qw);
-    });
+    return launch(
+        c, KID_LINEAR,
+        [&] {
+            const int thr = 64 * ((d.nz + 63) / 64);
+            k_lin_solve<<<dim3(d.nslot, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
+                                                                 c->WL, c->sWl, d, members(c, ids), c->T0t, c->U, c->DU,
+                                                                 c->DDU, c->sU, c->qw);
+        },
+        cap);
 }
 
 void free_ctx(rpcf_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->Wlin, c->F, c->sv, c->se, c->nl[0], c->nl[1], c->nl[2], c->nl[3], c->Y, c->W, c->Z2, c->minv,
-                    c->T0, c->T2, c->T4, c->U, c->dU, c->ddU, c->meanU, c->qw, c->tw, c->T0t, c->T2t, c->T4q, c->T2q,
-                    c->Tf};
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    void* ptrs[] = {c->F,   c->SV, c->SE, c->NL,  c->Y,     c->W,  c->Z2, c->WL, c->MINV, c->TF,  c->T4Q,
+                    c->T2Q, c->U,  c->DU, c->DDU, c->MEANU, c->MP, c->PH, c->T0, c->T2,   c->T4,  c->T0t,
+                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -377,43 +493,11 @@ void allow_lds(K kernel, int bytes) {
         (void)hipGetLastError();
 }
 
-}  // namespace
-
-// ======================================================================================
-// C ABI
-// ======================================================================================
-extern "C" {
-
-int rpcf_sizes(const rpcf_params* prm, int* nx, int* ny, int* nz, int* pz, int* hnx) {
-    Dims d;
-    int rc = check_sizes(prm, &d);
-    if (rc) return rc;
-    if (nx) *nx = d.nx;
-    if (ny) *ny = d.ny;
-    if (nz) *nz = d.nz;
-    if (pz) *pz = d.pz;
-    if (hnx) *hnx = d.hnx;
-    return RPCF_OK;
-}
-
-const char* rpcf_last_error(const rpcf_ctx* c) { return c ? c->err : "null context"; }
-
-void rpcf_destroy(rpcf_ctx* c) { free_ctx(c); }
-
-}  // extern "C"
-
-namespace {
-
-// allocation and set-up of a context; on failure the caller frees it
-int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
+// allocation and shape set-up of a context of `capacity` slots; on failure the caller frees it
+int init_ctx(rpcf_ctx* c) {
     const Dims d = c->d;
+    const long C = c->capacity;
     int rc;
-    // problem(RPCF_Paras): include/data.h:321-322
-    c->lp.aphi = prm->n_pi_x / 2;
-    c->lp.beta = prm->n_pi_y / 2;
-    c->lp.Re = prm->Re;
-    c->lp.dt = prm->dt;
-    c->Ro = prm->Ro;
     c->fac = 1.0 / d.mx / d.my;  // normalize factor, source/transform.cu:267
     HIPCHK(c, hipGetDevice(&c->device));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -421,79 +505,52 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lim > 0)
         c->lds_limit = lim;
 
-    const size_t P = d.P, nz = d.nz;
-    if ((rc = dalloc(c, &c->F, 6 * P * nz))) return rc;
-    if ((rc = dalloc(c, &c->sv, P * nz))) return rc;
-    if ((rc = dalloc(c, &c->se, P * nz))) return rc;
-    for (int i = 0; i < 4; i++)
-        if ((rc = dalloc(c, &c->nl[i], P * nz))) return rc;
-    if ((rc = dalloc(c, &c->Y, 6 * (size_t)d.hnx * d.pz * d.ny))) return rc;
-    if ((rc = dalloc(c, &c->W, 6 * (size_t)d.pz * d.my * d.nxh))) return rc;
+    const SlotSizes ss = slot_sizes(d);
+    c->sF = ss.F; c->sS = ss.S; c->sNL = ss.NL; c->sY = ss.Y; c->sW = ss.W; c->sZ = ss.Z;
+    c->sWl = ss.Wl; c->sM = ss.M; c->sTf = ss.Tf; c->sQ = ss.Q; c->sU = ss.U; c->sPz = ss.Pz;
     c->zld = (d.hnx + 7) & ~7;
-    if ((rc = dalloc(c, &c->Z2, 3 * (size_t)d.pz * d.my * c->zld))) return rc;
-    if ((rc = dalloc(c, &c->minv, 2 * (size_t)d.nslot * nz * nz))) return rc;
-    if ((rc = dalloc(c, &c->Wlin, 5 * P * nz))) return rc;
-    if ((rc = dalloc(c, &c->T0, nz * nz)) || (rc = dalloc(c, &c->T2, nz * nz)) || (rc = dalloc(c, &c->T4, nz * nz)))
+    c->tv_rt = (d.nz + 15) / 16;
+    c->tv_s = (d.nz + 3) / 4;
+    if ((rc = dalloc(c, &c->F, C * ss.F)) || (rc = dalloc(c, &c->SV, C * ss.S)) || (rc = dalloc(c, &c->SE, C * ss.S)) ||
+        (rc = dalloc(c, &c->NL, C * ss.NL)) || (rc = dalloc(c, &c->Y, C * ss.Y)) || (rc = dalloc(c, &c->W, C * ss.W)) ||
+        (rc = dalloc(c, &c->Z2, C * ss.Z)) || (rc = dalloc(c, &c->WL, C * ss.Wl)) ||
+        (rc = dalloc(c, &c->MINV, C * ss.M)) || (rc = dalloc(c, &c->TF, C * ss.Tf)) ||
+        (rc = dalloc(c, &c->T4Q, C * ss.Q)) || (rc = dalloc(c, &c->T2Q, C * ss.Q)) ||
+        (rc = dalloc(c, &c->U, C * ss.U)) || (rc = dalloc(c, &c->DU, C * ss.U)) || (rc = dalloc(c, &c->DDU, C * ss.U)) ||
+        (rc = dalloc(c, &c->MEANU, C * ss.Pz)) || (rc = dalloc(c, &c->MP, (size_t)C)) ||
+        (rc = dalloc(c, &c->PH, (size_t)C)) || (rc = dalloc(c, &c->ids_dev, (size_t)C)) ||
+        (rc = dalloc(c, &c->ids_zero, 1)))
         return rc;
-    if ((rc = dalloc(c, &c->T0t, nz * nz)) || (rc = dalloc(c, &c->T2t, nz * nz)) || (rc = dalloc(c, &c->T4q, nz * nz)) ||
-        (rc = dalloc(c, &c->T2q, nz * nz)))
-        return rc;
-    if ((rc = dalloc(c, &c->U, nz)) || (rc = dalloc(c, &c->dU, nz)) || (rc = dalloc(c, &c->ddU, nz)) ||
-        (rc = dalloc(c, &c->qw, nz)) || (rc = dalloc(c, &c->meanU, (size_t)d.pz)))
+    const size_t nz = d.nz;
+    if ((rc = dalloc(c, &c->T0, nz * nz)) || (rc = dalloc(c, &c->T2, nz * nz)) || (rc = dalloc(c, &c->T4, nz * nz)) ||
+        (rc = dalloc(c, &c->T0t, nz * nz)) || (rc = dalloc(c, &c->T2t, nz * nz)) || (rc = dalloc(c, &c->qw, nz)))
         return rc;
     if ((rc = dalloc(c, &c->tw, 2 * ((size_t)d.mx + d.my + 4 * d.N + 2 * d.N)))) return rc;
-    for (int i = 0; i < 4; i++) HIPCHK(c, hipMemset(c->nl[i], 0, sizeof(double2) * P * nz));
-    HIPCHK(c, hipMemset(c->F, 0, sizeof(double2) * 6 * P * nz));
-    HIPCHK(c, hipMemset(c->sv, 0, sizeof(double2) * P * nz));
-    HIPCHK(c, hipMemset(c->se, 0, sizeof(double2) * P * nz));
+    HIPCHK(c, hipMemset(c->ids_zero, 0, sizeof(int)));
+    HIPCHK(c, hipMemset(c->PH, 0, sizeof(int) * C));
+    c->slots.assign(C, SlotHost());
 
-    // operators (initAuxMatrix, source/solver.cpp:265-269)
-    std::vector<double> T0(nz * nz), T2(nz * nz), T4(nz * nz), U(nz), dU(nz), ddU(nz), mU(d.pz);
-    rpcf_host::t_matrices(d.N, T0.data(), T2.data(), T4.data());
-    rpcf_host::base_profile(d.nz, prm->u_profile, U.data(), dU.data(), ddU.data());
-    rpcf_host::base_profile_padded(d.pz, prm->u_profile, mU.data());
+    // shape operators (initAuxMatrix, source/solver.cpp:265-269)
+    c->T0h.resize(nz * nz);
+    c->T2h.resize(nz * nz);
+    c->T4h.resize(nz * nz);
+    rpcf_host::t_matrices(d.N, c->T0h.data(), c->T2h.data(), c->T4h.data());
     c->qw_host.resize(nz);
     for (size_t k = 0; k < nz; k++) c->qw_host[k] = (k % 2 == 0) ? 2.0 / (1.0 - (double)k * (double)k) : 0.0;
-    HIPCHK(c, hipMemcpy(c->T0, T0.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->T2, T2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->T4, T4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->T0, c->T0h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->T2, c->T2h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->T4, c->T4h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
     {
-        // transposed T0, and the quotients T2*0.5/Re*dt and T4*dt*0.5/Re of
-        // source/linear.cpp:100, 142 evaluated in the reference's order
-        std::vector<double> t0(nz * nz), t2(nz * nz), q4(nz * nz), q2(nz * nz);
-        const double Re = prm->Re, dt = prm->dt;
+        std::vector<double> t0(nz * nz), t2(nz * nz);
         for (size_t i = 0; i < nz; i++)
             for (size_t j = 0; j < nz; j++) {
-                t0[j * nz + i] = T0[i * nz + j];
-                t2[j * nz + i] = T2[i * nz + j];
-                q2[j * nz + i] = T2[i * nz + j] * 0.5 / Re * dt;
-                q4[j * nz + i] = T4[i * nz + j] * dt * 0.5 / Re;
+                t0[j * nz + i] = c->T0h[i * nz + j];
+                t2[j * nz + i] = c->T2h[i * nz + j];
             }
         HIPCHK(c, hipMemcpy(c->T0t, t0.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->T2t, t2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->T4q, q4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->T2q, q2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-        // the five operators (T0, T2, T4q for v; T0, T2q for eta) as MFMA A-operand tiles
-        const int RT = (int)((nz + 15) / 16), S = (int)((nz + 3) / 4);
-        c->tv_rt = RT;
-        c->tv_s = S;
-        const std::vector<double>* ops[5] = {&t0, &t2, &q4, &t0, &q2};  // transposed: op[j*nz + i] = T[i][j]
-        std::vector<double> tf((size_t)5 * RT * S * 64, 0.0);
-        for (int m = 0; m < 5; m++)
-            for (int rt = 0; rt < RT; rt++)
-                for (int sk = 0; sk < S; sk++)
-                    for (int l = 0; l < 64; l++) {
-                        const size_t i = 16 * rt + (l & 15), k = 4 * sk + (l >> 4);
-                        if (i < nz && k < nz) tf[(((size_t)m * RT + rt) * S + sk) * 64 + l] = (*ops[m])[k * nz + i];
-                    }
-        if ((rc = dalloc(c, &c->Tf, tf.size()))) return rc;
-        HIPCHK(c, hipMemcpy(c->Tf, tf.data(), sizeof(double) * tf.size(), hipMemcpyHostToDevice));
     }
-    HIPCHK(c, hipMemcpy(c->U, U.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->dU, dU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->ddU, ddU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->qw, c->qw_host.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->meanU, mU.data(), sizeof(double) * d.pz, hipMemcpyHostToDevice));
 
     // FFT plans (the reference's cuFFT plans, source/transform.cu:49-71)
     double* tw = c->tw;
@@ -505,9 +562,8 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     tw += 2 * 4 * d.N;
     if ((rc = make_plan(c, 2 * d.N, tw, &c->pl_zn, &c->cap_zn))) return rc;
 
-    // LDS tiles
+    // LDS tiles, sized for three resident blocks per CU
     lim = c->lds_limit;
-    // Chebyshev kernels: tiles sized for three resident blocks per CU
     const int lim3 = lim / 3;
     c->ld_zn = 2 * d.N + 2;  // >= 2N (FFT) and >= 2 nz (L_x, L_y of k_p2s_nl)
     c->ld_my = d.my + 1;
@@ -543,10 +599,77 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     allow_lds(k_lin_tv<kTvPencils>, lim);
     (void)hipGetLastError();
     if (getenv("RPCF_VERBOSE"))
-        fprintf(stderr, "rpcf: lds_limit=%d TB_s2p=%d TB_nl=%d TK=%d RP=%d nslot=%d\n", lim, c->TB_s2p, c->TB_nl,
-                c->TK_inv, c->RP, d.nslot);
+        fprintf(stderr, "rpcf: lds_limit=%d TB_s2p=%d TB_nl=%d TK=%d RP=%d nslot=%d capacity=%ld\n", lim, c->TB_s2p,
+                c->TB_nl, c->TK_inv, c->RP, d.nslot, C);
+    return RPCF_OK;
+}
 
-    // collocation matrices and inverses, in chunks of slots (initAuxMatrix)
+// Physics of one slot: base profile, per-slot operator tables and the inverses of its
+// Crank-Nicolson collocation matrices (initAuxMatrix, source/solver.cpp:260-320); the slot's
+// state, fields and history are cleared.
+int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
+    const Dims d = c->d;
+    const size_t nz = d.nz;
+    int rc;
+    SlotHost& sh = c->slots[slot];
+    sh = SlotHost();
+    sh.prm = *prm;
+    // problem(RPCF_Paras): include/data.h:321-322
+    LinParams lp;
+    lp.aphi = prm->n_pi_x / 2;
+    lp.beta = prm->n_pi_y / 2;
+    lp.Re = prm->Re;
+    lp.dt = prm->dt;
+    sh.mp.aphi = lp.aphi;
+    sh.mp.beta = lp.beta;
+    sh.mp.Re = lp.Re;
+    sh.mp.dt = lp.dt;
+    sh.mp.Ro = prm->Ro;
+    sh.mp.q_re = sh.mp.q_im = 0.0;
+    sh.mp.flow_control = prm->flow_control;
+
+    std::vector<double> U(nz), dU(nz), ddU(nz), mU(d.pz);
+    rpcf_host::base_profile(d.nz, prm->u_profile, U.data(), dU.data(), ddU.data());
+    rpcf_host::base_profile_padded(d.pz, prm->u_profile, mU.data());
+    HIPCHK(c, hipMemcpy(c->U + slot * c->sU, U.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->DU + slot * c->sU, dU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->DDU + slot * c->sU, ddU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->MEANU + slot * c->sPz, mU.data(), sizeof(double) * d.pz, hipMemcpyHostToDevice));
+    {
+        // the quotients T2*0.5/Re*dt and T4*dt*0.5/Re of source/linear.cpp:100, 142 in the
+        // reference's order (transposed), and the five operators of the right-hand sides as
+        // MFMA A-operand tiles (T0, T2, T4q for v; T0, T2q for eta)
+        const double Re = prm->Re, dt = prm->dt;
+        std::vector<double> t0(nz * nz), t2(nz * nz), q4(nz * nz), q2(nz * nz);
+        for (size_t i = 0; i < nz; i++)
+            for (size_t j = 0; j < nz; j++) {
+                t0[j * nz + i] = c->T0h[i * nz + j];
+                t2[j * nz + i] = c->T2h[i * nz + j];
+                q2[j * nz + i] = c->T2h[i * nz + j] * 0.5 / Re * dt;
+                q4[j * nz + i] = c->T4h[i * nz + j] * dt * 0.5 / Re;
+            }
+        HIPCHK(c, hipMemcpy(c->T4Q + slot * c->sQ, q4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->T2Q + slot * c->sQ, q2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        const int RT = c->tv_rt, S = c->tv_s;
+        const std::vector<double>* ops[5] = {&t0, &t2, &q4, &t0, &q2};  // op[j*nz + i] = T[i][j]
+        std::vector<double> tf((size_t)c->sTf, 0.0);
+        for (int m = 0; m < 5; m++)
+            for (int rt = 0; rt < RT; rt++)
+                for (int sk = 0; sk < S; sk++)
+                    for (int l = 0; l < 64; l++) {
+                        const size_t i = 16 * rt + (l & 15), k = 4 * sk + (l >> 4);
+                        if (i < nz && k < nz) tf[(((size_t)m * RT + rt) * S + sk) * 64 + l] = (*ops[m])[k * nz + i];
+                    }
+        HIPCHK(c, hipMemcpy(c->TF + slot * c->sTf, tf.data(), sizeof(double) * tf.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipMemcpy(c->MP + slot, &sh.mp, sizeof(MemberParams), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->PH + slot, 0, sizeof(int)));
+    HIPCHK(c, hipMemset(c->NL + slot * c->sNL, 0, sizeof(double2) * c->sNL));
+    HIPCHK(c, hipMemset(c->F + slot * c->sF, 0, sizeof(double2) * c->sF));
+    HIPCHK(c, hipMemset(c->SV + slot * c->sS, 0, sizeof(double2) * c->sS));
+    HIPCHK(c, hipMemset(c->SE + slot * c->sS, 0, sizeof(double2) * c->sS));
+
+    // collocation matrices and inverses, in chunks of matrix slots (initAuxMatrix)
     const size_t per = nz * nz;
     const int chunk =
         (int)std::min<size_t>((size_t)d.nslot, std::max<size_t>(64, ((size_t)1 << 29) / (2 * per * sizeof(double2))));
@@ -555,10 +678,11 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     if ((rc = dalloc(c, &work, 2 * (size_t)chunk * per))) return rc;
     if ((rc = dalloc(c, &status, 1))) { (void)hipFree(work); return rc; }
     (void)hipMemset(status, 0, sizeof(int));
+    double2* minv = c->MINV + slot * c->sM;
     for (int s0 = 0; s0 < d.nslot && !rc; s0 += chunk) {
         dim3 g(std::min(chunk, d.nslot - s0), 2);
-        k_build_inverse<<<g, 256, 0, c->stream>>>(c->minv, work, s0, d, c->lp, c->T0, c->T2, c->T4, c->U, c->ddU,
-                                                  status);
+        k_build_inverse<<<g, 256, 0, c->stream>>>(minv, work, s0, d, lp, c->T0, c->T2, c->T4, c->U + slot * c->sU,
+                                                  c->DDU + slot * c->sU, status);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(c, RPCF_EDEVICE, "k_build_inverse: %s", hipGetErrorString(e));
     }
@@ -569,24 +693,105 @@ int init_ctx(rpcf_ctx* c, const rpcf_params* prm) {
     (void)hipFree(work);
     (void)hipFree(status);
     if (!rc && hstatus) rc = fail(c, RPCF_EINVAL, "%d singular collocation matrices", hstatus);
+    if (!rc) sh.configured = 1;
     return rc;
 }
 
-}  // namespace
+int same_shape(const rpcf_params* a, const rpcf_params* b) { return a->mx == b->mx && a->my == b->my && a->mz == b->mz; }
 
-extern "C" {
+int set_state_slot(rpcf_ctx* c, int slot, const double* v_hat, const double* eta_hat, long stride) {
+    int rc = upload_spec(c, c->SV + slot * c->sS, v_hat, stride);
+    if (!rc) rc = upload_spec(c, c->SE + slot * c->sS, eta_hat, stride);
+    if (rc) return rc;
+    SlotHost& sh = c->slots[slot];
+    sh.have_state = 1;
+    // initFlow tail (source/RPCFKernels.cu:413-427); the flow-rate target is the initial bulk of u0
+    double qre = 0.0, qim = 0.0;
+    for (int k = 0; k < c->d.nz; k++) {
+        const double* s = v_hat + 2 * k;
+        qre = qre + s[0] * c->qw_host[k];
+        qim = qim + s[1] * c->qw_host[k];
+    }
+    sh.mp.q_re = qre;
+    sh.mp.q_im = qim;
+    HIPCHK(c, hipMemcpy(c->MP + slot, &sh.mp, sizeof(MemberParams), hipMemcpyHostToDevice));
+    sh.fields = FIELDS_STALE;
+    return RPCF_OK;
+}
 
-int rpcf_create(const rpcf_params* prm, rpcf_ctx** out) {
+// nsteps grouped steps of the listed slots: velocity of the state, nonlinear, linear;
+// replayed from a captured graph when graphs are enabled (and timing is off)
+int group_steps(rpcf_ctx* c, const std::vector<int>& ids, int nsteps) {
+    int rc = set_ids(c, ids);
+    if (rc) return rc;
+    const int n = (int)ids.size();
+    const bool graph = c->use_graphs && !c->timing;
+    if (graph && (!c->gexec || c->g_ids != ids)) {
+        if (c->gexec) {
+            (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enq_nonlinear(c, c->ids_dev, n, true, true);
+        if (!rc) rc = enq_linear(c, c->ids_dev, n, true);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, RPCF_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->gexec = nullptr;
+            return fail(c, RPCF_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
+        }
+        c->g_ids = ids;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        if (graph) {
+            HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            rc = enq_nonlinear(c, c->ids_dev, n, true);
+            if (!rc) rc = enq_linear(c, c->ids_dev, n);
+            if (rc) return rc;
+        }
+        for (int id : ids) c->slots[id].phase ^= 1;
+    }
+    for (int id : ids) c->slots[id].fields = FIELDS_STALE;
+    return sync_ctx(c);
+}
+
+int check_list(rpcf_ctx* c, const int* slots, int n, std::vector<int>* out) {
+    if (!c) return RPCF_EINVAL;
+    if (!slots || n < 1 || n > c->capacity) return fail(c, RPCF_EINVAL, "bad slot list (n = %d)", n);
+    std::vector<int> ids(slots, slots + n);
+    std::vector<int> sorted(ids);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(c, RPCF_EINVAL, "slot listed twice");
+    for (int s : ids) {
+        int rc = check_slot(c, s);
+        if (rc) return rc;
+        if (!c->slots[s].have_state) return fail(c, RPCF_ESTATE, "slot %d: no state set", s);
+    }
+    *out = ids;
+    return RPCF_OK;
+}
+
+int create_ctx(const rpcf_params* shape, int capacity, rpcf_ctx** out) {
     if (!out) return RPCF_EINVAL;
     *out = nullptr;
     Dims d;
-    if (check_sizes(prm, &d)) return RPCF_EINVAL;
+    if (check_sizes(shape, &d) || capacity < 1) return RPCF_EINVAL;
     rpcf_ctx* c = new rpcf_ctx();
-    c->prm = *prm;
+    c->shape = *shape;
     c->d = d;
-    int rc = init_ctx(c, prm);
+    c->capacity = capacity;
+    int rc = init_ctx(c);
     if (rc) {
-        fprintf(stderr, "rpcf_create: %s\n", c->err);
+        fprintf(stderr, "rpcf: %s\n", c->err);
         free_ctx(c);
         return rc;
     }
@@ -594,116 +799,148 @@ int rpcf_create(const rpcf_params* prm, rpcf_ctx** out) {
     return RPCF_OK;
 }
 
-int rpcf_set_state(rpcf_ctx* c, const double* v_hat, const double* eta_hat, long stride) {
-    if (!c) return RPCF_EINVAL;
-    int rc = upload_spec(c, c->sv, v_hat, stride);
-    if (!rc) rc = upload_spec(c, c->se, eta_hat, stride);
+}  // namespace
+
+// ======================================================================================
+// C ABI: one simulation (slot 0)
+// ======================================================================================
+extern "C" {
+
+int rpcf_sizes(const rpcf_params* prm, int* nx, int* ny, int* nz, int* pz, int* hnx) {
+    Dims d;
+    int rc = check_sizes(prm, &d);
     if (rc) return rc;
-    c->have_state = 1;
-    // initFlow tail (source/RPCFKernels.cu:413-427); the flow-rate target is the initial bulk of u0
-    double2 q = make_double2(0.0, 0.0);
-    for (int k = 0; k < c->d.nz; k++) {
-        const double* s = v_hat + 2 * k;
-        q = make_double2(q.x + s[0] * c->qw_host[k], q.y + s[1] * c->qw_host[k]);
-    }
-    c->q_target = q;
-    c->fields = FIELDS_STALE;
+    if (nx) *nx = d.nx;
+    if (ny) *ny = d.ny;
+    if (nz) *nz = d.nz;
+    if (pz) *pz = d.pz;
+    if (hnx) *hnx = d.hnx;
     return RPCF_OK;
 }
 
+const char* rpcf_last_error(const rpcf_ctx* c) { return c ? c->err : "null context"; }
+
+void rpcf_destroy(rpcf_ctx* c) { free_ctx(c); }
+
+int rpcf_create(const rpcf_params* prm, rpcf_ctx** out) {
+    int rc = create_ctx(prm, 1, out);
+    if (rc) return rc;
+    rc = init_slot(*out, 0, prm);
+    if (rc) {
+        fprintf(stderr, "rpcf_create: %s\n", (*out)->err);
+        free_ctx(*out);
+        *out = nullptr;
+    }
+    return rc;
+}
+
+int rpcf_set_state(rpcf_ctx* c, const double* v_hat, const double* eta_hat, long stride) {
+    int rc = check_slot(c, 0);
+    return rc ? rc : set_state_slot(c, 0, v_hat, eta_hat, stride);
+}
+
 int rpcf_get_state(rpcf_ctx* c, double* v_hat, double* eta_hat, long stride) {
-    if (!c) return RPCF_EINVAL;
-    int rc = download_spec(c, v_hat, c->sv, stride);
-    return rc ? rc : download_spec(c, eta_hat, c->se, stride);
+    int rc = check_slot(c, 0);
+    if (!rc) rc = download_spec(c, v_hat, c->SV, stride);
+    return rc ? rc : download_spec(c, eta_hat, c->SE, stride);
 }
 
 int rpcf_set_field(rpcf_ctx* c, int field, const double* f_hat, long stride) {
-    if (!c) return RPCF_EINVAL;
+    int rc = check_slot(c, 0);
+    if (rc) return rc;
     if (field < 0 || field > 5) return fail(c, RPCF_EINVAL, "bad field id %d", field);
-    if (c->fields == FIELDS_STALE && c->have_state) {
+    SlotHost& sh = c->slots[0];
+    if (sh.fields == FIELDS_STALE && sh.have_state) {
         // the other five fields keep their values: materialise them first
-        int rc = enq_velocity(c);
-        if (rc) return rc;
+        if ((rc = enq_velocity(c, c->ids_zero, 1))) return rc;
     }
-    int rc = upload_spec(c, c->F + (size_t)field * spec_elems(c), f_hat, stride);
+    rc = upload_spec(c, c->F + (size_t)field * spec_elems(c), f_hat, stride);
     if (!rc) {
-        c->have_state = 1;
-        c->fields = FIELDS_EXPLICIT;
+        sh.have_state = 1;
+        sh.fields = FIELDS_EXPLICIT;
     }
     return rc;
 }
 
 int rpcf_get_field(rpcf_ctx* c, int field, double* f_hat, long stride) {
-    if (!c) return RPCF_EINVAL;
+    int rc = check_slot(c, 0);
+    if (rc) return rc;
     if (field < 0 || field > 5) return fail(c, RPCF_EINVAL, "bad field id %d", field);
-    if (c->fields == FIELDS_STALE && c->have_state) {
-        int rc = enq_velocity(c);
-        if (rc) return rc;
-        c->fields = FIELDS_FRESH;
+    SlotHost& sh = c->slots[0];
+    if (sh.fields == FIELDS_STALE && sh.have_state) {
+        if ((rc = enq_velocity(c, c->ids_zero, 1))) return rc;
+        sh.fields = FIELDS_FRESH;
     }
     return download_spec(c, f_hat, c->F + (size_t)field * spec_elems(c), stride);
 }
 
 int rpcf_get_nonlinear(rpcf_ctx* c, int prev, double* n_v, double* n_eta, long stride) {
-    if (!c) return RPCF_EINVAL;
-    int rc = download_spec(c, n_v, c->nl[prev ? 1 : 0], stride);
-    return rc ? rc : download_spec(c, n_eta, c->nl[prev ? 3 : 2], stride);
+    int rc = check_slot(c, 0);
+    if (!rc) rc = download_spec(c, n_v, hist(c, 0, 0, prev), stride);
+    return rc ? rc : download_spec(c, n_eta, hist(c, 0, 1, prev), stride);
 }
 
 int rpcf_set_nonlinear(rpcf_ctx* c, int prev, const double* n_v, const double* n_eta, long stride) {
-    if (!c) return RPCF_EINVAL;
-    int rc = upload_spec(c, c->nl[prev ? 1 : 0], n_v, stride);
-    return rc ? rc : upload_spec(c, c->nl[prev ? 3 : 2], n_eta, stride);
+    int rc = check_slot(c, 0);
+    if (!rc) rc = upload_spec(c, hist(c, 0, 0, prev), n_v, stride);
+    return rc ? rc : upload_spec(c, hist(c, 0, 1, prev), n_eta, stride);
 }
 
 int rpcf_nonlinear(rpcf_ctx* c) {
-    if (!c) return RPCF_EINVAL;
-    if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_nonlinear: no state set");
-    int rc = enq_nonlinear(c);
-    return rc ? rc : sync_ctx(c);
+    int rc = check_slot(c, 0);
+    if (rc) return rc;
+    SlotHost& sh = c->slots[0];
+    if (!sh.have_state) return fail(c, RPCF_ESTATE, "rpcf_nonlinear: no state set");
+    const bool vel = sh.fields == FIELDS_STALE;
+    rc = enq_nonlinear(c, c->ids_zero, 1, vel);
+    if (rc) return rc;
+    if (vel) sh.fields = FIELDS_FRESH;
+    sh.phase ^= 1;
+    return sync_ctx(c);
 }
 
 int rpcf_linear(rpcf_ctx* c) {
-    if (!c) return RPCF_EINVAL;
-    if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_linear: no state set");
+    int rc = check_slot(c, 0);
+    if (rc) return rc;
+    SlotHost& sh = c->slots[0];
+    if (!sh.have_state) return fail(c, RPCF_ESTATE, "rpcf_linear: no state set");
     // the fields keep the velocity of the state before the solve until rpcf_velocity
-    if (c->fields == FIELDS_STALE) {
-        int rc = enq_velocity(c);
-        if (rc) return rc;
-    }
-    int rc = enq_linear(c);
-    c->fields = FIELDS_EXPLICIT;
+    if (sh.fields == FIELDS_STALE && (rc = enq_velocity(c, c->ids_zero, 1))) return rc;
+    rc = enq_linear(c, c->ids_zero, 1);
+    sh.fields = FIELDS_EXPLICIT;
     return rc ? rc : sync_ctx(c);
 }
 
 int rpcf_velocity(rpcf_ctx* c) {
-    if (!c) return RPCF_EINVAL;
-    if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_velocity: no state set");
-    int rc = enq_velocity(c);
-    c->fields = FIELDS_FRESH;
+    int rc = check_slot(c, 0);
+    if (rc) return rc;
+    SlotHost& sh = c->slots[0];
+    if (!sh.have_state) return fail(c, RPCF_ESTATE, "rpcf_velocity: no state set");
+    rc = enq_velocity(c, c->ids_zero, 1);
+    sh.fields = FIELDS_FRESH;
     return rc ? rc : sync_ctx(c);
 }
 
 int rpcf_step(rpcf_ctx* c, int nsteps) {
-    if (!c) return RPCF_EINVAL;
-    if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_step: no state set");
+    int rc = check_slot(c, 0);
+    if (rc) return rc;
+    SlotHost& sh = c->slots[0];
+    if (!sh.have_state) return fail(c, RPCF_ESTATE, "rpcf_step: no state set");
     // nextStep + getUVW: the velocity of the new state is formed when it is next needed
     // (by the next nonlinear evaluation, or by rpcf_get_field)
     for (int s = 0; s < nsteps; s++) {
-        int rc = enq_nonlinear(c);
-        if (!rc) rc = enq_linear(c);
+        rc = enq_nonlinear(c, c->ids_zero, 1, sh.fields == FIELDS_STALE);
+        if (!rc) rc = enq_linear(c, c->ids_zero, 1);
         if (rc) return rc;
-        c->fields = FIELDS_STALE;
+        sh.phase ^= 1;
+        sh.fields = FIELDS_STALE;
     }
     return sync_ctx(c);
 }
 
 int rpcf_bootstrap(rpcf_ctx* c) {
-    if (!c) return RPCF_EINVAL;
-    if (!c->have_state) return fail(c, RPCF_ESTATE, "rpcf_bootstrap: no state set");
-    int rc = enq_nonlinear(c);
-    if (rc) return rc;
-    return rpcf_step(c, 1);
+    int rc = rpcf_nonlinear(c);
+    return rc ? rc : rpcf_step(c, 1);
 }
 
 int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride, int padded) {
@@ -725,11 +962,12 @@ int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride
     if (!rc) rc = upload_spec(c, f, spec, stride);
     if (!rc) {
         dim3 g((d.ny + TB - 1) / TB, d.hnx, 1);
-        if (padded) k_cheb_s2p<1><<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(f, 0, y, 0, d, plz, tw4, TB, ld);
-        else k_cheb_s2p<0><<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(f, 0, y, 0, d, plz, tw4, TB, ld);
+        const size_t lds = sizeof(double2) * TB * ld;
+        if (padded) k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(f, 0, 0, y, 0, 0, d, plz, tw4, TB, ld, c->ids_zero);
+        else k_cheb_s2p<0><<<g, kFftThreads, lds, c->stream>>>(f, 0, 0, y, 0, 0, d, plz, tw4, TB, ld, c->ids_zero);
         dim3 g2((d.nxh + c->TK_inv - 1) / c->TK_inv, planes, 1);
-        k_span_inv<0><<<g2, kFftThreads, sizeof(double2) * c->TK_inv * c->ld_my, c->stream>>>(y, 0, w, 0, d, planes,
-                                                                                           c->pl_y, c->TK_inv, c->ld_my);
+        k_span_inv<0><<<g2, kFftThreads, sizeof(double2) * c->TK_inv * c->ld_my, c->stream>>>(
+            y, 0, 0, w, 0, 0, d, planes, c->pl_y, c->TK_inv, c->ld_my, c->ids_zero);
         dim3 g3((d.my / 2 + RP1 - 1) / RP1, planes);
         k_x_c2r<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(w, ph, d, c->pl_x, RP1, c->ld_mx);
         hipError_t e = hipGetLastError();
@@ -766,8 +1004,8 @@ int rpcf_phys_to_spec(rpcf_ctx* c, const double* phys, double* spec, long stride
         k_x_r2c<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(ph, z, d.hnx, c->fac, d, c->pl_x, RP1,
                                                                                   c->ld_mx);
         dim3 g2((d.hnx + c->TK_fwd - 1) / c->TK_fwd, planes, 1);
-        k_span_fwd<0><<<g2, kFftThreads, sizeof(double2) * c->TK_fwd * c->ld_my, c->stream>>>(z, 0, d.hnx, y, 0, d, planes,
-                                                                                           c->pl_y, c->TK_fwd, c->ld_my);
+        k_span_fwd<0><<<g2, kFftThreads, sizeof(double2) * c->TK_fwd * c->ld_my, c->stream>>>(
+            z, 0, 0, d.hnx, y, 0, 0, d, planes, c->pl_y, c->TK_fwd, c->ld_my, c->ids_zero);
         dim3 g((d.ny + TB - 1) / TB, d.hnx);
         k_cheb_p2s<<<g, kFftThreads, sizeof(double2) * TB * ld, c->stream>>>(y, f, d, planes, plz, TB, ld);
         hipError_t e = hipGetLastError();
@@ -781,14 +1019,16 @@ int rpcf_phys_to_spec(rpcf_ctx* c, const double* phys, double* spec, long stride
 
 int rpcf_get_inverse(rpcf_ctx* c, int eq, int pencil, double* out) {
     if (!c || !out || eq < 0 || eq > 1 || pencil < 0 || pencil >= c->d.P) return RPCF_EINVAL;
+    int rc = check_slot(c, 0);
+    if (rc) return rc;
     const Dims d = c->d;
     const int kx = pencil % d.hnx, ky = pencil / d.hnx;
     const int kyh = ky <= d.ny / 2 ? ky : d.ny - ky;
     const size_t nn = (size_t)d.nz * d.nz;
     std::vector<double2> m(nn);
-    int rc = sync_ctx(c);
+    rc = sync_ctx(c);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpy(m.data(), c->minv + ((size_t)eq * d.nslot + (size_t)kyh * d.hnx + kx) * nn,
+    HIPCHK(c, hipMemcpy(m.data(), c->MINV + ((size_t)eq * d.nslot + (size_t)kyh * d.hnx + kx) * nn,
                         sizeof(double2) * nn, hipMemcpyDeviceToHost));
     for (int r = 0; r < d.nz; r++)
         for (int j = 0; j < d.nz; j++) {
@@ -868,5 +1108,82 @@ int rpcf_timing_get(rpcf_ctx* c, int kid, double* total_ms, long* count) {
 const char* rpcf_kernel_name(int kid) { return (kid >= 0 && kid < KID_COUNT) ? kKernelNames[kid] : nullptr; }
 
 int rpcf_num_kernels(void) { return KID_COUNT; }
+
+// ======================================================================================
+// C ABI: groups (include/rpcf_group.h)
+// ======================================================================================
+int rpcf_group_create(const rpcf_params* shape, int capacity, rpcf_ctx** out) {
+    return create_ctx(shape, capacity, out);
+}
+
+int rpcf_group_capacity(const rpcf_ctx* c) { return c ? c->capacity : 0; }
+
+int rpcf_slot_bytes(const rpcf_params* shape, size_t* bytes) {
+    Dims d;
+    if (!bytes || check_sizes(shape, &d)) return RPCF_EINVAL;
+    *bytes = slot_sizes(d).bytes();
+    return RPCF_OK;
+}
+
+int rpcf_member_init(rpcf_ctx* c, int slot, const rpcf_params* prm) {
+    if (!c || !prm) return RPCF_EINVAL;
+    if (slot < 0 || slot >= c->capacity) return fail(c, RPCF_EINVAL, "slot %d outside 0..%d", slot, c->capacity - 1);
+    Dims d;
+    if (check_sizes(prm, &d) || !same_shape(prm, &c->shape))
+        return fail(c, RPCF_EINVAL, "member parameters do not match the group's grid shape");
+    int rc = sync_ctx(c);
+    if (rc) return rc;
+    c->slots[slot].configured = 0;
+    return init_slot(c, slot, prm);
+}
+
+int rpcf_member_release(rpcf_ctx* c, int slot) {
+    int rc = check_slot(c, slot);
+    if (rc) return rc;
+    rc = sync_ctx(c);
+    c->slots[slot] = SlotHost();
+    return rc;
+}
+
+int rpcf_member_set_state(rpcf_ctx* c, int slot, const double* v_hat, const double* eta_hat, long stride) {
+    int rc = check_slot(c, slot);
+    return rc ? rc : set_state_slot(c, slot, v_hat, eta_hat, stride);
+}
+
+int rpcf_member_get_state(rpcf_ctx* c, int slot, double* v_hat, double* eta_hat, long stride) {
+    int rc = check_slot(c, slot);
+    if (!rc) rc = download_spec(c, v_hat, c->SV + slot * c->sS, stride);
+    return rc ? rc : download_spec(c, eta_hat, c->SE + slot * c->sS, stride);
+}
+
+int rpcf_group_bootstrap(rpcf_ctx* c, const int* slots, int n) {
+    std::vector<int> ids;
+    int rc = check_list(c, slots, n, &ids);
+    if (rc) return rc;
+    if ((rc = set_ids(c, ids))) return rc;
+    rc = enq_nonlinear(c, c->ids_dev, n, true);
+    if (rc) return rc;
+    for (int id : ids) c->slots[id].phase ^= 1;
+    return group_steps(c, ids, 1);
+}
+
+int rpcf_group_step(rpcf_ctx* c, const int* slots, int n, int nsteps) {
+    std::vector<int> ids;
+    int rc = check_list(c, slots, n, &ids);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(c, RPCF_EINVAL, "negative step count");
+    return group_steps(c, ids, nsteps);
+}
+
+int rpcf_group_graphs(rpcf_ctx* c, int enable) {
+    if (!c) return RPCF_EINVAL;
+    int rc = sync_ctx(c);
+    c->use_graphs = enable ? 1 : 0;
+    if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    return rc;
+}
 
 }  // extern "C"

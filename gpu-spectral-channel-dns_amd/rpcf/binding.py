@@ -1,4 +1,4 @@
-"""ctypes binding of the rpcf C ABI (include/rpcf.h).
+"""ctypes binding of the rpcf C ABI (include/rpcf.h, include/rpcf_group.h).
 
 The same binding drives the HIP library (librpcf_hip.so) and the CPU FP64
 oracle (librpcf_oracle.so); both export the identical symbol set, so a parity
@@ -100,8 +100,25 @@ def load(path):
                                            ctypes.c_double, _dp, _dp, ctypes.c_long]),
         "rpcf_get_inverse": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     }
-    for name, (res, args) in sig.items():
-        fn = getattr(dll, name)
+    ip = ctypes.POINTER(ctypes.c_int)
+    group_sig = {  # HIP library only
+        "rpcf_group_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(_vp)]),
+        "rpcf_group_capacity": (ctypes.c_int, [_vp]),
+        "rpcf_slot_bytes": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_size_t)]),
+        "rpcf_member_init": (ctypes.c_int, [_vp, ctypes.c_int, P]),
+        "rpcf_member_release": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "rpcf_member_set_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_long]),
+        "rpcf_member_get_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_long]),
+        "rpcf_group_bootstrap": (ctypes.c_int, [_vp, ip, ctypes.c_int]),
+        "rpcf_group_step": (ctypes.c_int, [_vp, ip, ctypes.c_int, ctypes.c_int]),
+        "rpcf_group_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    }
+    for name, (res, args) in list(sig.items()) + list(group_sig.items()):
+        fn = getattr(dll, name, None)
+        if fn is None:
+            if name in group_sig:
+                continue
+            raise RpcfError(f"{path} does not export {name}")
         fn.restype = res
         fn.argtypes = args
     _CACHE[path] = dll
@@ -239,3 +256,86 @@ class Simulation:
         self._check(self.dll.rpcf_get_inverse(self.h, int(eq), int(pencil), _ptr(out.view(np.float64))),
                     "get_inverse")
         return out
+
+
+def slot_bytes(dll, prm):
+    """device bytes one group slot of this grid shape occupies"""
+    b = ctypes.c_size_t()
+    if dll.rpcf_slot_bytes(ctypes.byref(prm), ctypes.byref(b)) != 0:
+        raise RpcfError("rpcf_slot_bytes: unsupported sizes")
+    return b.value
+
+
+class Group:
+    """A group of simulations of one grid shape advanced by grouped launches
+    (include/rpcf_group.h, HIP library only)."""
+
+    def __init__(self, lib_path, shape, capacity):
+        self.dll = load(lib_path)
+        self.shape = shape
+        self.s = sizes(self.dll, shape)
+        h = _vp()
+        rc = self.dll.rpcf_group_create(ctypes.byref(shape), int(capacity), ctypes.byref(h))
+        if rc != 0:
+            raise RpcfError(f"rpcf_group_create failed rc={rc}")
+        self.h = h
+        self.capacity = int(capacity)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.dll.rpcf_last_error(self.h)
+            raise RpcfError(f"{what}: rc={rc}: {msg.decode() if msg else ''}")
+
+    def _spec(self):
+        return np.zeros((self.s["P"], self.s["nz"]), np.complex128)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.dll.rpcf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init(self, slot, prm):
+        self._check(self.dll.rpcf_member_init(self.h, int(slot), ctypes.byref(prm)), "member_init")
+
+    def release(self, slot):
+        self._check(self.dll.rpcf_member_release(self.h, int(slot)), "member_release")
+
+    def set_state(self, slot, v, eta):
+        v = np.ascontiguousarray(v, np.complex128)
+        eta = np.ascontiguousarray(eta, np.complex128)
+        self._check(self.dll.rpcf_member_set_state(self.h, int(slot), _ptr(v.view(np.float64)),
+                                                   _ptr(eta.view(np.float64)), v.shape[1]), "member_set_state")
+
+    def get_state(self, slot):
+        v, e = self._spec(), self._spec()
+        self._check(self.dll.rpcf_member_get_state(self.h, int(slot), _ptr(v.view(np.float64)),
+                                                   _ptr(e.view(np.float64)), self.s["nz"]), "member_get_state")
+        return v, e
+
+    @staticmethod
+    def _ids(slots):
+        a = (ctypes.c_int * len(slots))(*[int(x) for x in slots])
+        return a, len(slots)
+
+    def bootstrap(self, slots):
+        a, n = self._ids(slots)
+        self._check(self.dll.rpcf_group_bootstrap(self.h, a, n), "group_bootstrap")
+
+    def step(self, slots, nsteps=1):
+        a, n = self._ids(slots)
+        self._check(self.dll.rpcf_group_step(self.h, a, n, int(nsteps)), "group_step")
+
+    def graphs(self, enable=True):
+        self._check(self.dll.rpcf_group_graphs(self.h, int(bool(enable))), "group_graphs")

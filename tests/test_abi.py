@@ -11,11 +11,13 @@ import pytest
 import rpcf
 from helpers import IC, params
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "rpcf.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+HEADER = os.path.join(INCLUDE, "rpcf.h")
+GROUP_HEADER = os.path.join(INCLUDE, "rpcf_group.h")
 
 
-def declared():
-    src = open(HEADER).read()
+def declared(header=HEADER):
+    src = open(header).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(rpcf_\w+)\s*\(", src)))
 
@@ -33,6 +35,29 @@ def test_library_exports_every_declared_symbol(which):
     dll = ctypes.CDLL(path)
     missing = [n for n in declared() if not hasattr(dll, n)]
     assert not missing, missing
+
+
+def test_hip_library_exports_the_group_api():
+    """include/rpcf_group.h is implemented by the HIP library (the oracle runs one
+    simulation per context)"""
+    dll = ctypes.CDLL(rpcf.HIP_LIB)
+    names = declared(GROUP_HEADER)
+    assert "rpcf_group_step" in names and "rpcf_member_init" in names
+    missing = [n for n in names if not hasattr(dll, n)]
+    assert not missing, missing
+
+
+def test_slot_bytes():
+    """device bytes of one slot: the per-slot arrays (the inverse pair dominates)"""
+    dll = rpcf.load(rpcf.HIP_LIB)
+    prm = params("C1")
+    b = rpcf.slot_bytes(dll, prm)
+    s = rpcf.sizes(dll, prm)
+    nslot = s["hnx"] * (s["ny"] // 2 + 1)
+    minv = 2 * nslot * s["nz"] ** 2 * 16
+    assert minv < b < 1.6 * minv
+    with pytest.raises(rpcf.RpcfError):
+        rpcf.slot_bytes(dll, params("C0", mz=130))
 
 
 @pytest.mark.parametrize("which", ["oracle", "hip"])
