@@ -111,6 +111,51 @@ def copy_probe(nbytes, reps):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def ensemble_leg(n, nsteps):
+    """n independent simulations of mixed grid sizes (FP64): one group per shape, the
+    groups concurrently, against the same members run one at a time"""
+    from rpcf import ensemble
+    ms = ensemble.mixed_members(n, seed=0)
+    wall, runs = ensemble.run_grouped(ms, nsteps)
+    alone_total, alone_each, _ = ensemble.run_alone(ms, nsteps)
+    per100_group = [r.seconds / nsteps * 100 for r in runs for _ in r.members]
+    per100_alone = [t / nsteps * 100 for t in alone_each]
+    gp = sum(ensemble.grid_points(m) for m in ms) * nsteps
+    log(f"ensemble {n} members x {nsteps} steps: grouped {wall:.3f}s, one at a time {alone_total:.3f}s")
+    return {
+        "members": n, "steps": nsteps, "dtype": "f64",
+        "groups": {f"{r.shape[0]}x{r.shape[1]}x{r.shape[2]}": len(r.members) for r in runs},
+        "group_s": {f"{r.shape[0]}x{r.shape[1]}x{r.shape[2]}": round(r.seconds, 4) for r in runs},
+        "one_at_a_time_s": {f"{r.shape[0]}x{r.shape[1]}x{r.shape[2]}": round(sum(alone_each[i] for i in r.members), 4)
+                            for r in runs},
+        "steps_per_s": round(n * nsteps / wall, 1),
+        "steps_per_s_one_at_a_time": round(n * nsteps / alone_total, 1),
+        "speedup": round(alone_total / wall, 2),
+        "grid_point_steps_per_s": round(gp / wall, 1),
+        "p50_s_per_100_steps": round(float(np.percentile(per100_group, 50)), 4),
+        "p99_s_per_100_steps": round(float(np.percentile(per100_group, 99)), 4),
+        "p99_s_per_100_steps_one_at_a_time": round(float(np.percentile(per100_alone, 99)), 4),
+    }
+
+
+def serving_leg(window, horizon, steps):
+    """Poisson requests at 20 req/s over `window` seconds (grids 32x33x32 .. 256x129x256,
+    `steps` = (min, max) step counts), continuous admission by free HBM, served until all
+    finish or `horizon` seconds"""
+    from rpcf import serving
+    reqs = serving.poisson_requests(rate=20.0, duration=window, seed=0, steps=steps)
+    srv = serving.Server()
+    t0 = time.perf_counter()
+    done = srv.run(reqs, horizon)
+    wall = time.perf_counter() - t0
+    out = serving.summarize(done, reqs, wall)
+    out.update({"window_s": window, "horizon_s": horizon, "wall_s": round(wall, 2), "rate_per_s": 20.0,
+                "steps_range": list(steps), "dtype": "f64",
+                "groups": {f"{k[0]}x{k[1]}x{k[2]}": [len(g.slots) for g in v] for k, v in srv.groups.items()}})
+    log(f"serving: {out['finished']}/{out['submitted']} finished in {wall:.1f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +165,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
+    ap.add_argument("--ensemble", type=int, default=64, metavar="N",
+                    help="members of the mixed-size ensemble leg (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--serve", type=float, default=10.0, metavar="SECONDS",
+                    help="arrival window of the serving leg (Poisson, 20 req/s; 0: skip)")
+    ap.add_argument("--serve-horizon", type=float, default=40.0, metavar="SECONDS",
+                    help="wall-time limit of the serving leg")
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=(10, 200), metavar=("MIN", "MAX"),
+                    help="step counts of the serving requests (uniform)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one simulation on one GPU; multi-GPU runs are not "
@@ -176,6 +230,9 @@ def main():
                 "frac": dk["frac"], "traffic": read_traffic(dom)}
     sim.close()
 
+    ens = ensemble_leg(args.ensemble, args.ensemble_steps) if args.ensemble else None
+    srv = serving_leg(args.serve, args.serve_horizon, tuple(args.serve_steps)) if args.serve > 0 else None
+
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -207,6 +264,8 @@ def main():
         "cpu_baseline": cpu,
         "parity_1step_vs_oracle": parity,
         "kernels": kernels,
+        "ensemble": ens,
+        "serving": srv,
     }
     print(json.dumps(out), flush=True)
 

@@ -229,13 +229,33 @@ int dalloc(rpcf_ctx* c, T** p, size_t count) {
     return RPCF_OK;
 }
 
+#define RCCHK(expr)                \
+    do {                           \
+        const int rc_ = (expr);    \
+        if (rc_) return rc_;       \
+    } while (0)
+
+// Copies and fills on the context's stream, waited for: nothing touches the legacy null
+// stream, so a context can be used while another thread captures a graph on its own stream.
+int copy_sync(rpcf_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPCF_OK;
+}
+
+int zero_sync(rpcf_ctx* c, void* dst, size_t bytes) {
+    HIPCHK(c, hipMemsetAsync(dst, 0, bytes, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPCF_OK;
+}
+
 int make_plan(rpcf_ctx* c, int n, double* tw_dev, FftPlan* pl, int* cap) {
     pl->n = n;
     if (!rpcf_host::factorize(n, pl->rad, &pl->nst)) return fail(c, RPCF_EINVAL, "FFT length %d not 2^a3^b5^c", n);
     *cap = fft_capacity(pl->rad, pl->nst, kFftThreads);
     std::vector<double> tw(2 * (size_t)n);
     rpcf_host::twiddles(n, tw.data());
-    HIPCHK(c, hipMemcpy(tw_dev, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, tw_dev, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
     pl->tw = tw_dev;
     return RPCF_OK;
 }
@@ -271,7 +291,7 @@ int download_spec(rpcf_ctx* c, double* dst, const double2* src, long stride) {
     std::vector<double2> h(spec_elems(c));
     int rc = sync_ctx(c);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpy(h.data(), src, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
+    RCCHK(copy_sync(c, h.data(), src, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
     for (size_t p = 0; p < (size_t)c->d.P; p++)
         for (int k = 0; k < nz; k++) {
             double* s = dst + 2 * ((size_t)stride * p + k);
@@ -526,8 +546,8 @@ int init_ctx(rpcf_ctx* c) {
         (rc = dalloc(c, &c->T0t, nz * nz)) || (rc = dalloc(c, &c->T2t, nz * nz)) || (rc = dalloc(c, &c->qw, nz)))
         return rc;
     if ((rc = dalloc(c, &c->tw, 2 * ((size_t)d.mx + d.my + 4 * d.N + 2 * d.N)))) return rc;
-    HIPCHK(c, hipMemset(c->ids_zero, 0, sizeof(int)));
-    HIPCHK(c, hipMemset(c->PH, 0, sizeof(int) * C));
+    RCCHK(zero_sync(c, c->ids_zero, sizeof(int)));
+    RCCHK(zero_sync(c, c->PH, sizeof(int) * C));
     c->slots.assign(C, SlotHost());
 
     // shape operators (initAuxMatrix, source/solver.cpp:265-269)
@@ -537,9 +557,9 @@ int init_ctx(rpcf_ctx* c) {
     rpcf_host::t_matrices(d.N, c->T0h.data(), c->T2h.data(), c->T4h.data());
     c->qw_host.resize(nz);
     for (size_t k = 0; k < nz; k++) c->qw_host[k] = (k % 2 == 0) ? 2.0 / (1.0 - (double)k * (double)k) : 0.0;
-    HIPCHK(c, hipMemcpy(c->T0, c->T0h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->T2, c->T2h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->T4, c->T4h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->T0, c->T0h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->T2, c->T2h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->T4, c->T4h.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
     {
         std::vector<double> t0(nz * nz), t2(nz * nz);
         for (size_t i = 0; i < nz; i++)
@@ -547,10 +567,10 @@ int init_ctx(rpcf_ctx* c) {
                 t0[j * nz + i] = c->T0h[i * nz + j];
                 t2[j * nz + i] = c->T2h[i * nz + j];
             }
-        HIPCHK(c, hipMemcpy(c->T0t, t0.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->T2t, t2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        RCCHK(copy_sync(c, c->T0t, t0.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        RCCHK(copy_sync(c, c->T2t, t2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
     }
-    HIPCHK(c, hipMemcpy(c->qw, c->qw_host.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->qw, c->qw_host.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
 
     // FFT plans (the reference's cuFFT plans, source/transform.cu:49-71)
     double* tw = c->tw;
@@ -631,10 +651,10 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     std::vector<double> U(nz), dU(nz), ddU(nz), mU(d.pz);
     rpcf_host::base_profile(d.nz, prm->u_profile, U.data(), dU.data(), ddU.data());
     rpcf_host::base_profile_padded(d.pz, prm->u_profile, mU.data());
-    HIPCHK(c, hipMemcpy(c->U + slot * c->sU, U.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->DU + slot * c->sU, dU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->DDU + slot * c->sU, ddU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->MEANU + slot * c->sPz, mU.data(), sizeof(double) * d.pz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->U + slot * c->sU, U.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->DU + slot * c->sU, dU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->DDU + slot * c->sU, ddU.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->MEANU + slot * c->sPz, mU.data(), sizeof(double) * d.pz, hipMemcpyHostToDevice));
     {
         // the quotients T2*0.5/Re*dt and T4*dt*0.5/Re of source/linear.cpp:100, 142 in the
         // reference's order (transposed), and the five operators of the right-hand sides as
@@ -648,8 +668,8 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
                 q2[j * nz + i] = c->T2h[i * nz + j] * 0.5 / Re * dt;
                 q4[j * nz + i] = c->T4h[i * nz + j] * dt * 0.5 / Re;
             }
-        HIPCHK(c, hipMemcpy(c->T4Q + slot * c->sQ, q4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->T2Q + slot * c->sQ, q2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        RCCHK(copy_sync(c, c->T4Q + slot * c->sQ, q4.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
+        RCCHK(copy_sync(c, c->T2Q + slot * c->sQ, q2.data(), sizeof(double) * nz * nz, hipMemcpyHostToDevice));
         const int RT = c->tv_rt, S = c->tv_s;
         const std::vector<double>* ops[5] = {&t0, &t2, &q4, &t0, &q2};  // op[j*nz + i] = T[i][j]
         std::vector<double> tf((size_t)c->sTf, 0.0);
@@ -660,14 +680,14 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
                         const size_t i = 16 * rt + (l & 15), k = 4 * sk + (l >> 4);
                         if (i < nz && k < nz) tf[(((size_t)m * RT + rt) * S + sk) * 64 + l] = (*ops[m])[k * nz + i];
                     }
-        HIPCHK(c, hipMemcpy(c->TF + slot * c->sTf, tf.data(), sizeof(double) * tf.size(), hipMemcpyHostToDevice));
+        RCCHK(copy_sync(c, c->TF + slot * c->sTf, tf.data(), sizeof(double) * tf.size(), hipMemcpyHostToDevice));
     }
-    HIPCHK(c, hipMemcpy(c->MP + slot, &sh.mp, sizeof(MemberParams), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->PH + slot, 0, sizeof(int)));
-    HIPCHK(c, hipMemset(c->NL + slot * c->sNL, 0, sizeof(double2) * c->sNL));
-    HIPCHK(c, hipMemset(c->F + slot * c->sF, 0, sizeof(double2) * c->sF));
-    HIPCHK(c, hipMemset(c->SV + slot * c->sS, 0, sizeof(double2) * c->sS));
-    HIPCHK(c, hipMemset(c->SE + slot * c->sS, 0, sizeof(double2) * c->sS));
+    RCCHK(copy_sync(c, c->MP + slot, &sh.mp, sizeof(MemberParams), hipMemcpyHostToDevice));
+    RCCHK(zero_sync(c, c->PH + slot, sizeof(int)));
+    RCCHK(zero_sync(c, c->NL + slot * c->sNL, sizeof(double2) * c->sNL));
+    RCCHK(zero_sync(c, c->F + slot * c->sF, sizeof(double2) * c->sF));
+    RCCHK(zero_sync(c, c->SV + slot * c->sS, sizeof(double2) * c->sS));
+    RCCHK(zero_sync(c, c->SE + slot * c->sS, sizeof(double2) * c->sS));
 
     // collocation matrices and inverses, in chunks of matrix slots (initAuxMatrix)
     const size_t per = nz * nz;
@@ -677,7 +697,7 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     int* status = nullptr;
     if ((rc = dalloc(c, &work, 2 * (size_t)chunk * per))) return rc;
     if ((rc = dalloc(c, &status, 1))) { (void)hipFree(work); return rc; }
-    (void)hipMemset(status, 0, sizeof(int));
+    if ((rc = zero_sync(c, status, sizeof(int)))) { (void)hipFree(work); (void)hipFree(status); return rc; }
     double2* minv = c->MINV + slot * c->sM;
     for (int s0 = 0; s0 < d.nslot && !rc; s0 += chunk) {
         dim3 g(std::min(chunk, d.nslot - s0), 2);
@@ -688,8 +708,7 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     }
     int hstatus = 0;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, RPCF_EDEVICE, "matrix build failed");
-    if (!rc && hipMemcpy(&hstatus, status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail(c, RPCF_EDEVICE, "matrix build status copy failed");
+    if (!rc) rc = copy_sync(c, &hstatus, status, sizeof(int), hipMemcpyDeviceToHost);
     (void)hipFree(work);
     (void)hipFree(status);
     if (!rc && hstatus) rc = fail(c, RPCF_EINVAL, "%d singular collocation matrices", hstatus);
@@ -714,7 +733,7 @@ int set_state_slot(rpcf_ctx* c, int slot, const double* v_hat, const double* eta
     }
     sh.mp.q_re = qre;
     sh.mp.q_im = qim;
-    HIPCHK(c, hipMemcpy(c->MP + slot, &sh.mp, sizeof(MemberParams), hipMemcpyHostToDevice));
+    RCCHK(copy_sync(c, c->MP + slot, &sh.mp, sizeof(MemberParams), hipMemcpyHostToDevice));
     sh.fields = FIELDS_STALE;
     return RPCF_OK;
 }
@@ -732,7 +751,7 @@ int group_steps(rpcf_ctx* c, const std::vector<int>& ids, int nsteps) {
             c->gexec = nullptr;
         }
         hipGraph_t g = nullptr;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
         rc = enq_nonlinear(c, c->ids_dev, n, true, true);
         if (!rc) rc = enq_linear(c, c->ids_dev, n, true);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -974,8 +993,7 @@ int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride
         if (e != hipSuccess) rc = fail(c, RPCF_EDEVICE, "spec_to_phys launch: %s", hipGetErrorString(e));
     }
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, RPCF_EDEVICE, "spec_to_phys sync");
-    if (!rc && hipMemcpy(phys, ph, sizeof(double) * planes * d.my * d.mx, hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail(c, RPCF_EDEVICE, "spec_to_phys copy");
+    if (!rc) rc = copy_sync(c, phys, ph, sizeof(double) * planes * d.my * d.mx, hipMemcpyDeviceToHost);
     for (void* p : {(void*)f, (void*)y, (void*)w, (void*)ph})
         if (p) (void)hipFree(p);
     return rc;
@@ -997,8 +1015,7 @@ int rpcf_phys_to_spec(rpcf_ctx* c, const double* phys, double* spec, long stride
     if (!rc) rc = dalloc(c, &y, (size_t)d.hnx * planes * d.ny);
     if (!rc) rc = dalloc(c, &z, (size_t)planes * d.my * d.hnx);
     if (!rc) rc = dalloc(c, &ph, (size_t)planes * d.my * d.mx);
-    if (!rc && hipMemcpy(ph, phys, sizeof(double) * planes * d.my * d.mx, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(c, RPCF_EDEVICE, "phys_to_spec copy");
+    if (!rc) rc = copy_sync(c, ph, phys, sizeof(double) * planes * d.my * d.mx, hipMemcpyHostToDevice);
     if (!rc) {
         dim3 g3((d.my / 2 + RP1 - 1) / RP1, planes);
         k_x_r2c<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(ph, z, d.hnx, c->fac, d, c->pl_x, RP1,
@@ -1028,7 +1045,7 @@ int rpcf_get_inverse(rpcf_ctx* c, int eq, int pencil, double* out) {
     std::vector<double2> m(nn);
     rc = sync_ctx(c);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpy(m.data(), c->MINV + ((size_t)eq * d.nslot + (size_t)kyh * d.hnx + kx) * nn,
+    RCCHK(copy_sync(c, m.data(), c->MINV + ((size_t)eq * d.nslot + (size_t)kyh * d.hnx + kx) * nn,
                         sizeof(double2) * nn, hipMemcpyDeviceToHost));
     for (int r = 0; r < d.nz; r++)
         for (int j = 0; j < d.nz; j++) {
