@@ -111,6 +111,29 @@ def copy_probe(nbytes, reps):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64):
+    """the same workload in the mixed-precision form (FP32 transform intermediates, x/span
+    FFTs and Lamb vector; FP64 Chebyshev transforms, operators and solves): time per step
+    and the relative field error against the FP64 run after the same steps"""
+    prm = rpcf.make_params(**C1)
+    prm.precision = 1
+    with rpcf.Simulation(rpcf.HIP_LIB, prm) as sim:
+        sim.set_state(v0, e0)
+        sim.bootstrap()
+        sim.step(warmup)
+        device_sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        device_sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        v, e = sim.get_state()
+    err = {"v": float(np.abs(v - final64[0]).max() / np.abs(final64[0]).max()),
+           "eta": float(np.abs(e - final64[1]).max() / np.abs(final64[1]).max())}
+    log(f"mixed: {ms:.4f} ms/step ({ms64 / ms:.2f}x FP64), error after {1 + warmup + steps} steps {err}")
+    return {"ms_per_step": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3), "steps_compared": 1 + warmup + steps,
+            "max_rel_error": err, "dtype": "f32 transforms / f64 solves"}
+
+
 def ensemble_leg(n, nsteps):
     """n independent simulations of mixed grid sizes (FP64): one group per shape, the
     groups concurrently, against the same members run one at a time"""
@@ -165,6 +188,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--ensemble", type=int, default=64, metavar="N",
                     help="members of the mixed-size ensemble leg (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
@@ -228,7 +252,9 @@ def main():
     dk = kernels[dom]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": dk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": dk["frac"], "traffic": read_traffic(dom)}
+    final64 = sim.get_state()
     sim.close()
+    mixed = None if args.no_mixed else mixed_leg(prm, v0, e0, args.warmup, args.steps, ms_step, final64)
 
     ens = ensemble_leg(args.ensemble, args.ensemble_steps) if args.ensemble else None
     srv = serving_leg(args.serve, args.serve_horizon, tuple(args.serve_steps)) if args.serve > 0 else None
@@ -264,6 +290,7 @@ def main():
         "cpu_baseline": cpu,
         "parity_1step_vs_oracle": parity,
         "kernels": kernels,
+        "mixed": mixed,
         "ensemble": ens,
         "serving": srv,
     }

@@ -27,13 +27,20 @@ constexpr int kLambPairs = 2;  // pairs of span rows per block of k_lamb_phys
 
 template <int V> struct IntC { static constexpr int value = V; };
 
+// complex of the storage precision of the transform intermediates (FP64, or FP32 in the
+// mixed-precision form), and conversions at the FP64 boundaries
+template <typename R> using cplx = typename cplx_t<R>::type;
+template <typename C> __device__ __forceinline__ C to_c(double2 a) { return cmake<C>(a.x, a.y); }
+template <typename C> __device__ __forceinline__ double2 to_d2(C a) { return make_double2(a.x, a.y); }
+
 // Move `total` elements through registers with K loads in flight per thread before any is
 // consumed: a loop that loads, waits and stores one element at a time pays one memory
 // latency per element.
 template <int K, typename Load, typename Use>
 __device__ __forceinline__ void staged(int total, int tid, int nthr, Load load, Use use) {
+    using V = decltype(load(0));
     for (int base = tid; base < total; base += K * nthr) {
-        double2 v[K];
+        V v[K];
 #pragma unroll
         for (int u = 0; u < K; u++) {
             const int i = base + u * nthr;
@@ -83,9 +90,9 @@ __device__ __forceinline__ void s2p_odd_ext(double2* s, double2 a, int k, int N,
     }
 }
 
-template <int PAD, int FN = 0>  // FN: compile-time 2N or 0
+template <int PAD, int FN = 0, typename RO = double>  // FN: compile-time 2N or 0; RO: precision of Y
 __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F, long fstride, long sF,
-                                                  double2* __restrict__ Y, long ystride, long sY, Dims d, FftPlan pl,
+                                                  cplx<RO>* __restrict__ Y, long ystride, long sY, Dims d, FftPlan pl,
                                                   const double2* __restrict__ tw4, int TB, int ld,
                                                   const int* __restrict__ ids) {
     RPCF_LDS(double2);
@@ -98,7 +105,7 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long pstride = (long)d.hnx * nz;
     const double2* in = F + f * fstride + ((long)ky0 * d.hnx + kx) * nz;
-    double2* out = Y + f * ystride + (long)kx * nout * d.ny + ky0;
+    cplx<RO>* out = Y + f * ystride + (long)kx * nout * d.ny + ky0;
     const int total = nb * nz;
     auto load = [&](int i) {
         const int b = i / nz, k = i - b * nz;
@@ -144,7 +151,7 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
             },
             [&](int i, double2 v) {
                 const int m = i / nb, b = i - m * nb;
-                out[(long)(PAD ? 2 * m + half : m) * d.ny + b] = v;
+                out[(long)(PAD ? 2 * m + half : m) * d.ny + b] = to_c<cplx<RO>>(v);
             });
         __syncthreads();
     }
@@ -157,39 +164,40 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
 // field.  Tile: TK kx columns of one
 // plane.  Y[f][kx][q][ky] -> W[f][q][s][kx] for kx < nxh (the x-Nyquist column is zero).
 // -------------------------------------------------------------------------------------
-template <int FN = 0>
-__global__ __launch_bounds__(256) void k_span_inv(const double2* __restrict__ Y, long ystride, long sY, double2* __restrict__ W,
+template <int FN = 0, typename R = double>  // R: precision of Y, W and the FFT
+__global__ __launch_bounds__(256) void k_span_inv(const cplx<R>* __restrict__ Y, long ystride, long sY, cplx<R>* __restrict__ W,
                                                   long wstride, long sW, Dims d, int planes, FftPlan pl, int TK, int ld,
                                                   const int* __restrict__ ids) {
-    RPCF_LDS(double2);
+    using C2 = cplx<R>;
+    RPCF_LDS(C2);
     const int f = blockIdx.z % 6, q = blockIdx.y, kx0 = blockIdx.x * TK;
     const long slot = ids[blockIdx.z / 6];
     Y += slot * sY;
     W += slot * sW;
     const int nk = min(TK, d.nxh - kx0);
     const int my = FN ? FN : d.my, tid = threadIdx.x, nthr = blockDim.x;
-    const double2* in = Y + f * ystride;
+    const C2* in = Y + f * ystride;
     staged<12>(
         nk * my, tid, nthr,
         [&](int i) {
             const int kxi = i / my, s = i - kxi * my;
             const int ky = ky_of_span(d, s);
-            return ky >= 0 ? in[((long)(kx0 + kxi) * planes + q) * d.ny + ky] : make_double2(0.0, 0.0);
+            return ky >= 0 ? in[((long)(kx0 + kxi) * planes + q) * d.ny + ky] : cmake<C2>(0, 0);
         },
-        [&](int i, double2 v) {
+        [&](int i, C2 v) {
             const int kxi = i / my, s = i - kxi * my;
             lds[kxi * ld + s] = v;
         });
     __syncthreads();
     fft_any<FN, +1, kSpanTile>(lds, ld, nk, pl, tid, nthr);
-    double2* out = W + f * wstride;
+    C2* out = W + f * wstride;
     staged<12>(
         my * nk, tid, nthr,
         [&](int i) {
             const int s = i / nk, kxi = i - s * nk;
             return lds[kxi * ld + s];
         },
-        [&](int i, double2 v) {
+        [&](int i, C2 v) {
             const int s = i / nk, kxi = i - s * nk;
             out[((long)q * my + s) * d.nxh + kx0 + kxi] = v;
         });
@@ -213,10 +221,13 @@ __device__ __forceinline__ double2 c2r_pack(const double2* rA, const double2* rB
 }
 
 // Split the FFT Z of a packed real pair z = a + i b into the half spectra of a and b.
-__device__ __forceinline__ void r2c_unpack(const double2* z, int k, int mx, double fac, double2* xa, double2* xb) {
-    const double2 zk = z[k], zm = z[k == 0 ? 0 : mx - k];
-    *xa = make_double2((zk.x + zm.x) * 0.5 * fac, (zk.y - zm.y) * 0.5 * fac);
-    *xb = make_double2((zk.y + zm.y) * 0.5 * fac, -(zk.x - zm.x) * 0.5 * fac);
+template <typename C>
+__device__ __forceinline__ void r2c_unpack(const C* z, int k, int mx, decltype(C().x) fac, C* xa, C* xb) {
+    using T = decltype(C().x);
+    const C zk = z[k], zm = z[k == 0 ? 0 : mx - k];
+    const T h = T(0.5);
+    *xa = cmake<C>((zk.x + zm.x) * h * fac, (zk.y - zm.y) * h * fac);
+    *xb = cmake<C>((zk.y + zm.y) * h * fac, -(zk.x - zm.x) * h * fac);
 }
 
 // -------------------------------------------------------------------------------------
@@ -227,15 +238,16 @@ __device__ __forceinline__ void r2c_unpack(const double2* z, int k, int mx, doub
 // padded base profile meanU).  One block: one plane q, RP
 // pairs of span rows.  W[6][pz][my][nxh] -> Z2[3][pz][my][hnx].
 // -------------------------------------------------------------------------------------
-template <int FN = 0>
-__global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W, long wstride, long sW,
-                                                   double2* __restrict__ Z2, long zstride, long sZ, int zld,
+template <int FN = 0, typename R = double>  // R: precision of W, Z2, the FFTs and the products
+__global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W, long wstride, long sW,
+                                                   cplx<R>* __restrict__ Z2, long zstride, long sZ, int zld,
                                                    const double* __restrict__ meanU, long sPz, double fac, Dims d,
                                                    FftPlan pl, int RP, int ld, Members mb) {
-    RPCF_LDS(double2);
+    using C2 = cplx<R>;
+    RPCF_LDS(C2);
     const int q = blockIdx.y, rp0 = blockIdx.x * RP;
     const int slot = mb.ids[blockIdx.z];
-    const double Ro = mb.mp[slot].Ro;
+    const R Ro = (R)mb.mp[slot].Ro;
     W += slot * sW;
     Z2 += slot * sZ;
     meanU += slot * sPz;
@@ -246,14 +258,14 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W
     // Z_{mx-k}; Z_k = 0 for nxh <= k <= mx - nxh
     constexpr int KL = 3;  // loads per thread in flight: 6 * kLambPairs * 64 / 256
     for (int base = tid; base < 6 * nrp * nxh; base += KL * nthr) {
-        double2 va[KL], vb[KL];
+        C2 va[KL], vb[KL];
 #pragma unroll
         for (int u = 0; u < KL; u++) {
             const int i = base + u * nthr;
             if (i < 6 * nrp * nxh) {
                 const int seq = i / nxh, k = i - seq * nxh;
                 const int f = seq / nrp, ip = seq - f * nrp;
-                const double2* rA = W + f * wstride + ((long)q * my + 2 * (rp0 + ip)) * nxh;
+                const C2* rA = W + f * wstride + ((long)q * my + 2 * (rp0 + ip)) * nxh;
                 va[u] = rA[k];
                 vb[u] = rA[nxh + k];
             }
@@ -263,36 +275,39 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W
             const int i = base + u * nthr;
             if (i < 6 * nrp * nxh) {
                 const int seq = i / nxh, k = i - seq * nxh;
-                double2 a = va[u], b = vb[u];
-                double2* z = lds + seq * ld;
+                C2 a = va[u], b = vb[u];
+                C2* z = lds + seq * ld;
                 if (k == 0) {
-                    a.y = 0.0;
-                    b.y = 0.0;
+                    a.y = 0;
+                    b.y = 0;
                 } else {
-                    z[mx - k] = make_double2(a.x + b.y, -a.y + b.x);
+                    z[mx - k] = cmake<C2>(a.x + b.y, -a.y + b.x);
                 }
-                z[k] = make_double2(a.x - b.y, a.y + b.x);
+                z[k] = cmake<C2>(a.x - b.y, a.y + b.x);
             }
         }
     }
     for (int i = tid; i < 6 * nrp * (mx - 2 * nxh + 1); i += nthr) {
         const int seq = i / (mx - 2 * nxh + 1), k = i - seq * (mx - 2 * nxh + 1);
-        lds[seq * ld + nxh + k] = make_double2(0.0, 0.0);
+        lds[seq * ld + nxh + k] = cmake<C2>(0, 0);
     }
     __syncthreads();
     fft_any<FN, +1, 6 * kLambPairs>(lds, ld, 6 * nrp, pl, tid, nthr);
-    const double mU = meanU[q];
+    // Ro U(y) is uniform in x and span, so it reaches only the (0,0) pencil of L_z, which
+    // rhsNonlinear does not use: the FP32 form leaves it out (in FP32 its round-off would
+    // swamp the perturbation products); the FP64 form keeps the reference's expression.
+    const R mU = sizeof(R) == 8 ? (R)meanU[q] : R(0);
     for (int i = tid; i < nrp * mx; i += nthr) {
         const int ip = i / mx, x = i - ip * mx;
-        const double2 u = lds[(0 * nrp + ip) * ld + x], v = lds[(1 * nrp + ip) * ld + x];
-        const double2 w = lds[(2 * nrp + ip) * ld + x], ox = lds[(3 * nrp + ip) * ld + x];
-        const double2 oy = lds[(4 * nrp + ip) * ld + x], oz = lds[(5 * nrp + ip) * ld + x];
-        const double lxa = oz.x * v.x - oy.x * w.x - Ro * w.x, lxb = oz.y * v.y - oy.y * w.y - Ro * w.y;
-        const double lya = ox.x * w.x - oz.x * u.x, lyb = ox.y * w.y - oz.y * u.y;
-        const double lza = oy.x * u.x - ox.x * v.x + Ro * (u.x + mU), lzb = oy.y * u.y - ox.y * v.y + Ro * (u.y + mU);
-        lds[(0 * nrp + ip) * ld + x] = make_double2(lxa, lxb);
-        lds[(1 * nrp + ip) * ld + x] = make_double2(lya, lyb);
-        lds[(2 * nrp + ip) * ld + x] = make_double2(lza, lzb);
+        const C2 u = lds[(0 * nrp + ip) * ld + x], v = lds[(1 * nrp + ip) * ld + x];
+        const C2 w = lds[(2 * nrp + ip) * ld + x], ox = lds[(3 * nrp + ip) * ld + x];
+        const C2 oy = lds[(4 * nrp + ip) * ld + x], oz = lds[(5 * nrp + ip) * ld + x];
+        const R lxa = oz.x * v.x - oy.x * w.x - Ro * w.x, lxb = oz.y * v.y - oy.y * w.y - Ro * w.y;
+        const R lya = ox.x * w.x - oz.x * u.x, lyb = ox.y * w.y - oz.y * u.y;
+        const R lza = oy.x * u.x - ox.x * v.x + Ro * (u.x + mU), lzb = oy.y * u.y - ox.y * v.y + Ro * (u.y + mU);
+        lds[(0 * nrp + ip) * ld + x] = cmake<C2>(lxa, lxb);
+        lds[(1 * nrp + ip) * ld + x] = cmake<C2>(lya, lyb);
+        lds[(2 * nrp + ip) * ld + x] = cmake<C2>(lza, lzb);
     }
     __syncthreads();
     fft_any<FN, -1, 3 * kLambPairs>(lds, ld, 3 * nrp, pl, tid, nthr);
@@ -300,9 +315,9 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const double2* __restrict__ W
     for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
         const int seq = i / hnx, k = i - seq * hnx;
         const int g = seq / nrp, ip = seq - g * nrp;
-        double2 xa, xb;
-        r2c_unpack(lds + seq * ld, k, mx, fac, &xa, &xb);
-        double2* o = Z2 + g * zstride + ((long)q * my + 2 * (rp0 + ip)) * zld;
+        C2 xa, xb;
+        r2c_unpack(lds + seq * ld, k, mx, (R)fac, &xa, &xb);
+        C2* o = Z2 + g * zstride + ((long)q * my + 2 * (rp0 + ip)) * zld;
         o[k] = xa;
         o[zld + k] = xb;
     }
@@ -361,38 +376,39 @@ __global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, 
 // (source/transpose.cu:223-257).  Z2[g][q][s][kx] -> Z1[g][kx][q][ky]; blockIdx.z =
 // member * 3 + component.
 // -------------------------------------------------------------------------------------
-template <int FN = 0>
-__global__ __launch_bounds__(256) void k_span_fwd(const double2* __restrict__ Z2, long zstride, long sZ, int zld,
-                                                  double2* __restrict__ Z1, long ostride, long sY, Dims d, int planes,
+template <int FN = 0, typename R = double>  // R: precision of Z2, Z1 and the FFT
+__global__ __launch_bounds__(256) void k_span_fwd(const cplx<R>* __restrict__ Z2, long zstride, long sZ, int zld,
+                                                  cplx<R>* __restrict__ Z1, long ostride, long sY, Dims d, int planes,
                                                   FftPlan pl, int TK, int ld, const int* __restrict__ ids) {
-    RPCF_LDS(double2);
+    using C2 = cplx<R>;
+    RPCF_LDS(C2);
     const int g = blockIdx.z % 3, q = blockIdx.y, kx0 = blockIdx.x * TK;
     const long slot = ids[blockIdx.z / 3];
     Z2 += slot * sZ;
     Z1 += slot * sY;
     const int nk = min(TK, d.hnx - kx0);
     const int my = FN ? FN : d.my, ny = d.ny, tid = threadIdx.x, nthr = blockDim.x;
-    const double2* in = Z2 + g * zstride;
+    const C2* in = Z2 + g * zstride;
     staged<12>(
         my * nk, tid, nthr,
         [&](int i) {
             const int s = i / nk, kxi = i - s * nk;
             return in[((long)q * my + s) * zld + kx0 + kxi];
         },
-        [&](int i, double2 v) {
+        [&](int i, C2 v) {
             const int s = i / nk, kxi = i - s * nk;
             lds[kxi * ld + s] = v;
         });
     __syncthreads();
     fft_any<FN, -1, kSpanTile>(lds, ld, nk, pl, tid, nthr);
-    double2* out = Z1 + g * ostride;
+    C2* out = Z1 + g * ostride;
     staged<8>(
         nk * ny, tid, nthr,
         [&](int i) {
             const int kxi = i / ny, ky = i - kxi * ny;
             return lds[kxi * ld + span_of(d, ky)];
         },
-        [&](int i, double2 v) {
+        [&](int i, C2 v) {
             const int kxi = i / ny, ky = i - kxi * ny;
             out[((long)(kx0 + kxi) * planes + q) * ny + ky] = v;
         });
@@ -501,8 +517,8 @@ __device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int
 // L_y come from cheb_deriv_scan.  LDS: FFT tile [TB][ld] (ld >= 2N+2, also holding L_x, L_y
 // as [TB][2][nz]) and 256 chunk totals.
 // -------------------------------------------------------------------------------------
-template <int FN = 0>  // FN: compile-time 2N or 0
-__global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, long zstride, long sY,
+template <int FN = 0, typename RI = double>  // FN: compile-time 2N or 0; RI: precision of Z1
+__global__ __launch_bounds__(256) void k_p2s_nl(const cplx<RI>* __restrict__ Z1, long zstride, long sY,
                                                 double2* __restrict__ NL, long sNL, Dims d, FftPlan plN,
                                                 const double2* __restrict__ tw4, int TB, int ld, Members mb) {
     RPCF_LDS(double2);
@@ -527,14 +543,14 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const double2* __restrict__ Z1, 
     // one Lamb component g (compile-time, so that ck[g][.] stays in registers)
     auto component = [&](auto gc) {
         constexpr int g = decltype(gc)::value;
-        const double2* in = Z1 + g * zstride + (long)kx * d.pz * ny + ky0;
+        const cplx<RI>* in = Z1 + g * zstride + (long)kx * d.pz * ny + ky0;
         for (int half = 0; half < 2; half++) {
             // samples f_{2m+half}: row 2m+half of the plane stack, nb consecutive ky
             staged<KC>(
                 (half ? N : N + 1) * nb, tid, nthr,
                 [&](int i) {
                     const int m = i / nb, b = i - m * nb;
-                    return in[(long)(2 * m + half) * ny + b];
+                    return to_d2(in[(long)(2 * m + half) * ny + b]);
                 },
                 [&](int i, double2 v) {
                     const int m = i / nb, b = i - m * nb;

@@ -65,7 +65,10 @@ struct rpcf_ctx {
     // tiles: sequences per block and LDS row stride (elements) of each transform kernel
     int TB_s2p = 1, TB_nl = 1, TK_inv = 1, TK_fwd = 1, RP = 1;
     int ld_zn = 0, ld_my = 0, ld_mx = 0, zld = 0;
+    int mixed = 0;                   // 1: transform intermediates and x/span FFTs in FP32
     FftPlan pl_x{}, pl_y{}, pl_zp{}, pl_zn{};
+    FftPlan pl_xf{}, pl_yf{};        // FP32 twiddles of the x and span plans (mixed form)
+    float* twf = nullptr;
     int cap_x = 0, cap_y = 0, cap_zp = 0, cap_zn = 0;  // fft_capacity of each plan
     int tv_rt = 0, tv_s = 0;                           // MFMA row tiles and k-steps of the operator products
     // per-slot arrays ([capacity][...]) and their slot strides in elements
@@ -130,7 +133,7 @@ int check_sizes(const rpcf_params* prm, Dims* out) {
     const int mx = prm->mx, my = prm->my, mz = prm->mz;
     if (mx < 6 || my < 6 || mz < 16 || mx % 2 || my % 2 || mz % 4) return RPCF_EINVAL;
     if (!(prm->Re > 0) || !(prm->dt > 0) || !(prm->n_pi_x > 0) || !(prm->n_pi_y > 0)) return RPCF_EINVAL;
-    if (prm->precision != 0) return RPCF_EINVAL;
+    if (prm->precision != 0 && prm->precision != 1) return RPCF_EINVAL;
     Dims d;
     d.mx = mx; d.my = my;
     d.nx = mx / 3 * 2; d.ny = my / 3 * 2;
@@ -345,15 +348,25 @@ int enq_velocity(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
         cap);
 }
 
-// getNonlinear for the listed slots; with_velocity: form their fields from the state first
-int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool cap = false) {
+// Backward transforms, Lamb vector, forward transforms and rhsNonlinear of the listed
+// slots with the intermediates Y, W, Z2 and the x/span FFTs in precision R (FP64, or FP32
+// in the mixed form); the Chebyshev transforms and everything that reaches the state are
+// FP64.
+template <typename R>
+int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     const Dims d = c->d;
     int rc;
     const long fS = (long)d.P * d.nz, yS = (long)d.hnx * d.pz * d.ny, wS = (long)d.pz * d.my * d.nxh,
                zS = (long)d.pz * d.my * c->zld;
     const double2* tw4 = (const double2*)c->pl_zp.tw;
     const Members mb = members(c, ids);
-    if (with_velocity && (rc = enq_velocity(c, ids, n, cap))) return rc;
+    using C2 = cplx<R>;
+    C2* Y = reinterpret_cast<C2*>(c->Y);
+    C2* W = reinterpret_cast<C2*>(c->W);
+    C2* Z2 = reinterpret_cast<C2*>(c->Z2);
+    constexpr bool f64 = sizeof(R) == 8;
+    const FftPlan& plx = f64 ? c->pl_x : c->pl_xf;
+    const FftPlan& ply = f64 ? c->pl_y : c->pl_yf;
     {
         const int TB = c->TB_s2p, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, 6 * n);
@@ -362,12 +375,13 @@ int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool c
             c, KID_S2P,
             [&] {
                 switch (2 * d.N) {
-#define RPCF_CASE(L) case L: k_cheb_s2p<1, L><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, c->Y, yS, c->sY, d, c->pl_zn, tw4, TB, ld, ids); break;
+#define RPCF_CASE(L) case L: k_cheb_s2p<1, L, R><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, Y, yS, c->sY, d, c->pl_zn, tw4, TB, ld, ids); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
-                        k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, c->Y, yS, c->sY, d, c->pl_zn,
-                                                                          tw4, TB, ld, ids);
+                        if constexpr (f64)
+                            k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, Y, yS, c->sY, d,
+                                                                              c->pl_zn, tw4, TB, ld, ids);
                 }
             },
             cap);
@@ -376,17 +390,18 @@ int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool c
     {
         const int TK = c->TK_inv, ld = c->ld_my;
         dim3 g((d.nxh + TK - 1) / TK, d.pz, 6 * n);
-        size_t lds = sizeof(double2) * TK * ld;
+        size_t lds = sizeof(C2) * TK * ld;
         rc = launch(
             c, KID_SPAN_INV,
             [&] {
                 switch (d.my) {
-#define RPCF_CASE(L) case L: k_span_inv<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->W, wS, c->sW, d, d.pz, c->pl_y, TK, ld, ids); break;
+#define RPCF_CASE(L) case L: k_span_inv<L, R><<<g, kFftThreads, lds, c->stream>>>(Y, yS, c->sY, W, wS, c->sW, d, d.pz, ply, TK, ld, ids); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
-                        k_span_inv<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->W, wS, c->sW, d, d.pz,
-                                                                       c->pl_y, TK, ld, ids);
+                        if constexpr (f64)
+                            k_span_inv<<<g, kFftThreads, lds, c->stream>>>(Y, yS, c->sY, W, wS, c->sW, d, d.pz, ply,
+                                                                           TK, ld, ids);
                 }
             },
             cap);
@@ -395,18 +410,19 @@ int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool c
     {
         const int RP = c->RP, ld = c->ld_mx;
         dim3 g((d.my / 2 + RP - 1) / RP, d.pz, n);
-        size_t lds = sizeof(double2) * 6 * RP * ld;
+        size_t lds = sizeof(C2) * 6 * RP * ld;
         rc = launch(
             c, KID_LAMB,
             [&] {
                 switch (d.mx) {
-#define RPCF_CASE(L) case L: k_lamb_phys<L><<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->sW, c->Z2, zS, c->sZ, c->zld, c->MEANU, c->sPz, c->fac, d, c->pl_x, RP, ld, mb); break;
+#define RPCF_CASE(L) case L: k_lamb_phys<L, R><<<g, kFftThreads, lds, c->stream>>>(W, wS, c->sW, Z2, zS, c->sZ, c->zld, c->MEANU, c->sPz, c->fac, d, plx, RP, ld, mb); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
-                        k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(c->W, wS, c->sW, c->Z2, zS, c->sZ, c->zld,
-                                                                        c->MEANU, c->sPz, c->fac, d, c->pl_x, RP, ld,
-                                                                        mb);
+                        if constexpr (f64)
+                            k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(W, wS, c->sW, Z2, zS, c->sZ, c->zld,
+                                                                            c->MEANU, c->sPz, c->fac, d, plx, RP, ld,
+                                                                            mb);
                 }
             },
             cap);
@@ -415,17 +431,18 @@ int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool c
     {
         const int TK = c->TK_fwd, ld = c->ld_my;
         dim3 g((d.hnx + TK - 1) / TK, d.pz, 3 * n);
-        size_t lds = sizeof(double2) * TK * ld;
+        size_t lds = sizeof(C2) * TK * ld;
         rc = launch(
             c, KID_SPAN_FWD,
             [&] {
                 switch (d.my) {
-#define RPCF_CASE(L) case L: k_span_fwd<L><<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->sZ, c->zld, c->Y, yS, c->sY, d, d.pz, c->pl_y, TK, ld, ids); break;
+#define RPCF_CASE(L) case L: k_span_fwd<L, R><<<g, kFftThreads, lds, c->stream>>>(Z2, zS, c->sZ, c->zld, Y, yS, c->sY, d, d.pz, ply, TK, ld, ids); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
-                        k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(c->Z2, zS, c->sZ, c->zld, c->Y, yS, c->sY, d,
-                                                                       d.pz, c->pl_y, TK, ld, ids);
+                        if constexpr (f64)
+                            k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(Z2, zS, c->sZ, c->zld, Y, yS, c->sY, d,
+                                                                           d.pz, ply, TK, ld, ids);
                 }
             },
             cap);
@@ -441,17 +458,25 @@ int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool c
             c, KID_P2S_NL,
             [&] {
                 switch (2 * d.N) {
-#define RPCF_CASE(L) case L: k_p2s_nl<L><<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->NL, c->sNL, d, c->pl_zn, tw4, TB, ld, mb); break;
+#define RPCF_CASE(L) case L: k_p2s_nl<L, R><<<g, kFftThreads, lds, c->stream>>>(Y, yS, c->sY, c->NL, c->sNL, d, c->pl_zn, tw4, TB, ld, mb); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
-                        k_p2s_nl<<<g, kFftThreads, lds, c->stream>>>(c->Y, yS, c->sY, c->NL, c->sNL, d, c->pl_zn, tw4,
-                                                                     TB, ld, mb);
+                        if constexpr (f64)
+                            k_p2s_nl<<<g, kFftThreads, lds, c->stream>>>(Y, yS, c->sY, c->NL, c->sNL, d, c->pl_zn, tw4,
+                                                                         TB, ld, mb);
                 }
             },
             cap);
     }
     return rc;
+}
+
+// getNonlinear for the listed slots; with_velocity: form their fields from the state first
+int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool cap = false) {
+    int rc;
+    if (with_velocity && (rc = enq_velocity(c, ids, n, cap))) return rc;
+    return c->mixed ? enq_transforms<float>(c, ids, n, cap) : enq_transforms<double>(c, ids, n, cap);
 }
 
 int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
@@ -496,7 +521,7 @@ void free_ctx(rpcf_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     void* ptrs[] = {c->F,   c->SV, c->SE, c->NL,  c->Y,     c->W,  c->Z2, c->WL, c->MINV, c->TF,  c->T4Q,
                     c->T2Q, c->U,  c->DU, c->DDU, c->MEANU, c->MP, c->PH, c->T0, c->T2,   c->T4,  c->T0t,
-                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero};
+                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -578,6 +603,21 @@ int init_ctx(rpcf_ctx* c) {
     tw += 2 * d.mx;
     if ((rc = make_plan(c, d.my, tw, &c->pl_y, &c->cap_y))) return rc;
     tw += 2 * d.my;
+    if (c->mixed) {
+        // FP32 copies of the x and span twiddles
+        std::vector<double> t64(2 * (size_t)d.mx), t64y(2 * (size_t)d.my);
+        rpcf_host::twiddles(d.mx, t64.data());
+        rpcf_host::twiddles(d.my, t64y.data());
+        std::vector<float> t32(2 * ((size_t)d.mx + d.my));
+        for (size_t i = 0; i < t64.size(); i++) t32[i] = (float)t64[i];
+        for (size_t i = 0; i < t64y.size(); i++) t32[t64.size() + i] = (float)t64y[i];
+        if ((rc = dalloc(c, &c->twf, t32.size()))) return rc;
+        RCCHK(copy_sync(c, c->twf, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+        c->pl_xf = c->pl_x;
+        c->pl_xf.tw = c->twf;
+        c->pl_yf = c->pl_y;
+        c->pl_yf.tw = c->twf + 2 * d.mx;
+    }
     if ((rc = make_plan(c, 4 * d.N, tw, &c->pl_zp, &c->cap_zp))) return rc;
     tw += 2 * 4 * d.N;
     if ((rc = make_plan(c, 2 * d.N, tw, &c->pl_zn, &c->cap_zn))) return rc;
@@ -610,7 +650,12 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_span_fwd<L>, lim);    \
     allow_lds(k_lamb_phys<L>, lim);   \
     allow_lds(k_p2s_nl<L>, lim);      \
-    allow_lds(k_lin_tv_mfma<(L / 2 + 4) / 4>, lim);
+    allow_lds(k_lin_tv_mfma<(L / 2 + 4) / 4>, lim);      \
+    allow_lds(k_cheb_s2p<1, L, float>, lim);              \
+    allow_lds(k_span_inv<L, float>, lim);                 \
+    allow_lds(k_span_fwd<L, float>, lim);                 \
+    allow_lds(k_lamb_phys<L, float>, lim);                \
+    allow_lds(k_p2s_nl<L, float>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     allow_lds(k_cheb_p2s, lim);
@@ -804,10 +849,19 @@ int create_ctx(const rpcf_params* shape, int capacity, rpcf_ctx** out) {
     *out = nullptr;
     Dims d;
     if (check_sizes(shape, &d) || capacity < 1) return RPCF_EINVAL;
+    if (shape->precision == 1) {
+        // the mixed form runs compiled FFT lengths only
+        bool ok_x = false, ok_y = false, ok_z = false;
+#define RPCF_CASE(L) ok_x |= d.mx == L; ok_y |= d.my == L; ok_z |= 2 * d.N == L;
+        RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+        if (!(ok_x && ok_y && ok_z)) return RPCF_EINVAL;
+    }
     rpcf_ctx* c = new rpcf_ctx();
     c->shape = *shape;
     c->d = d;
     c->capacity = capacity;
+    c->mixed = shape->precision == 1;
     int rc = init_ctx(c);
     if (rc) {
         fprintf(stderr, "rpcf: %s\n", c->err);
@@ -1055,6 +1109,46 @@ int rpcf_get_inverse(rpcf_ctx* c, int eq, int pencil, double* out) {
     return RPCF_OK;
 }
 
+// Bound of the peak perturbation velocity of a spectral state: per velocity component (u, v,
+// w of getVelocityKernel_sm, source/velocity.cu:90-170) the sum of the magnitudes of its
+// coefficients over all pencils, the conjugate pencil of kx > 0 counted (|T_k| <= 1), and
+// the largest of the three.  D v by the descending recurrence, b_0 halved.  The oracle
+// computes the same sums in the same order, so both libraries seed identical states.
+static double seed_velocity_bound(const rpcf_params* prm, const double* v, const double* e, long stride, int nz,
+                                  int ny, int hnx) {
+    const double aphi = prm->n_pi_x / 2, beta = prm->n_pi_y / 2;
+    double su = 0.0, sv = 0.0, sw = 0.0;
+    std::vector<double> b(2 * (size_t)nz + 4);
+    for (int ky = 0; ky < ny; ky++)
+        for (int kx = 0; kx < hnx; kx++) {
+            if (kx == 0 && ky == 0) continue;
+            const size_t p = (size_t)ky * hnx + kx;
+            const double* a = v + 2 * (size_t)stride * p;
+            const double* h = e + 2 * (size_t)stride * p;
+            const double mult = kx == 0 ? 1.0 : 2.0;
+            const double al = (double)kx / aphi;
+            const double be = (ky >= ny / 2 + 1 ? (double)(ky - ny) : (double)ky) / beta;
+            const double k2 = al * al + be * be;
+            std::fill(b.begin(), b.end(), 0.0);
+            for (int k = nz - 2; k >= 0; k--) {
+                b[2 * k] = b[2 * (k + 2)] + 2.0 * (k + 1) * a[2 * (k + 1)];
+                b[2 * k + 1] = b[2 * (k + 2) + 1] + 2.0 * (k + 1) * a[2 * (k + 1) + 1];
+            }
+            b[0] = b[0] * 0.5;
+            b[1] = b[1] * 0.5;
+            for (int k = 0; k < nz; k++) {
+                const double ur = -(al * b[2 * k + 1] - be * h[2 * k + 1]) / k2;
+                const double ui = (al * b[2 * k] - be * h[2 * k]) / k2;
+                const double wr = -(be * b[2 * k + 1] + al * h[2 * k + 1]) / k2;
+                const double wi = (be * b[2 * k] + al * h[2 * k]) / k2;
+                su = su + mult * sqrt(ur * ur + ui * ui);
+                sv = sv + mult * sqrt(a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1]);
+                sw = sw + mult * sqrt(wr * wr + wi * wi);
+            }
+        }
+    return std::max(std::max(su, sv), sw);
+}
+
 int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int kmax, int degmax, double kexp,
                     double* v_hat, double* eta_hat, long stride) {
     Dims d;
@@ -1073,7 +1167,7 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
             if (kx == nx / 2 || ky == ny / 2) continue;
             if (kx > kmax || std::abs(kys) > kmax) continue;
             if (kx == 0 && kys < 0) continue;
-            const double scale = amplitude * pow(1.0 + sqrt((double)kx * kx + (double)kys * kys), -kexp);
+            const double scale = pow(1.0 + sqrt((double)kx * kx + (double)kys * kys), -kexp);
             const size_t p = (size_t)ky * hnx + kx;
             for (int field = 0; field < 2; field++) {
                 std::fill(g.begin(), g.end(), 0.0);
@@ -1099,6 +1193,16 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
                 a[2 * ((size_t)stride * pm + k) + 1] = -a[2 * ((size_t)stride * pp + k) + 1];
             }
         }
+    }
+    // amplitude: the bound of the peak perturbation velocity
+    const double bound = seed_velocity_bound(prm, v_hat, eta_hat, stride, nz, ny, hnx);
+    if (bound > 0.0) {
+        const double f = amplitude / bound;
+        for (size_t p = 0; p < P; p++)
+            for (int k = 0; k < 2 * nz; k++) {
+                v_hat[2 * (size_t)stride * p + k] *= f;
+                eta_hat[2 * (size_t)stride * p + k] *= f;
+            }
     }
     return RPCF_OK;
 }
@@ -1146,8 +1250,8 @@ int rpcf_member_init(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     if (!c || !prm) return RPCF_EINVAL;
     if (slot < 0 || slot >= c->capacity) return fail(c, RPCF_EINVAL, "slot %d outside 0..%d", slot, c->capacity - 1);
     Dims d;
-    if (check_sizes(prm, &d) || !same_shape(prm, &c->shape))
-        return fail(c, RPCF_EINVAL, "member parameters do not match the group's grid shape");
+    if (check_sizes(prm, &d) || !same_shape(prm, &c->shape) || prm->precision != c->shape.precision)
+        return fail(c, RPCF_EINVAL, "member parameters do not match the group's grid shape and precision");
     int rc = sync_ctx(c);
     if (rc) return rc;
     c->slots[slot].configured = 0;

@@ -149,7 +149,7 @@ class Server:
     capacity(shape, slot_bytes) -> slots per new group of that shape; a group is created
     when its bytes fit in the free HBM minus `reserve` bytes."""
 
-    def __init__(self, lib=B.HIP_LIB, chunk=10, reserve=8 << 30, capacity=None, max_groups_per_shape=4):
+    def __init__(self, lib=B.HIP_LIB, chunk=10, reserve=8 << 30, capacity=None, max_groups_per_shape=2):
         self.lib = lib
         self.chunk = chunk
         self.reserve = reserve

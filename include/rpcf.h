@@ -45,10 +45,12 @@ typedef struct rpcf_ctx rpcf_ctx;
  *                 U = (1+y)/2 of source/coefficient.cu:149-159 and
  *                 source/nonlinear.cu:400-403; {0.5,0.5,0} is the reference.
  *   flow_control  0: none (reference); 1: constant flow rate -- a uniform mean
- *                 pressure-gradient correction keeps the bulk of u0 at zero.
+ *                 pressure-gradient correction keeps the bulk of u0 at its value at set_state.
  *   precision     0: FP64 everywhere (reference REAL_DOUBLE, include/data.h:11);
- *                 1: mixed -- FP32 x/z FFTs and Lamb vector, split-precision FP32
- *                 Chebyshev transforms, FP64 solves (HIP library only). */
+ *                 1: mixed (HIP library, compiled FFT lengths only) -- the transform
+ *                 intermediates, the x and span FFTs and the Lamb vector in FP32; the
+ *                 Chebyshev transforms, the nonlinear combination, the operators and the
+ *                 solves in FP64.  The oracle computes FP64 whatever this field says. */
 typedef struct rpcf_params {
     int mx, my, mz;
     double n_pi_x, n_pi_y;
@@ -140,11 +142,14 @@ int rpcf_get_inverse(rpcf_ctx* ctx, int eq, int pencil, double* out);
 
 /* Seeded wall-satisfying initial field (host only, no device work): complex Gaussian
  * Chebyshev coefficients of degree <= degmax on pencils with kx <= kmax and
- * |signed ky| <= kmax (pencil (0,0) excluded), times amplitude * (1+|k|)^(-kexp) with
+ * |signed ky| <= kmax (pencil (0,0) excluded), times (1+|k|)^(-kexp) with
  * |k| = sqrt(kx^2 + ky^2) in index units, multiplied by (1-y^2)^2 for v and (1-y^2)
- * for eta, Hermitian on kx = 0, Nyquist pencils zero.  Generator: splitmix64 +
- * Box-Muller, draws in pencil order.  No counterpart in the reference (its
- * init_flow_kernel analytic field, source/RPCFKernels.cu:221-356, is replaced). */
+ * for eta, Hermitian on kx = 0, Nyquist pencils zero; then scaled so that `amplitude`
+ * bounds the peak perturbation velocity: for each of u, v, w the sum of the magnitudes of
+ * its spectral coefficients (which bounds its maximum over the domain) is at most
+ * `amplitude`.  Generator: splitmix64 + Box-Muller, draws in pencil order.  No
+ * counterpart in the reference (its init_flow_kernel analytic field,
+ * source/RPCFKernels.cu:221-356, is replaced). */
 int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int kmax,
                     int degmax, double kexp, double* v_hat, double* eta_hat, long pencil_stride);
 

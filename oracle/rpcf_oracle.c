@@ -1151,6 +1151,48 @@ static void mul_one_minus_y2(const cx* a, cx* b, int n) {
     free(yya);
 }
 
+/* Bound of the peak perturbation velocity of a spectral state: per velocity component
+ * (u, v, w of getVelocityKernel_sm, source/velocity.cu:90-170), the sum of the magnitudes of
+ * its Chebyshev-Fourier coefficients over all pencils, the conjugate pencil of kx > 0
+ * counted (|T_k| <= 1, so this bounds the maximum over the domain); the largest of the
+ * three.  D v by the descending recurrence b_{k} = b_{k+2} + 2(k+1) a_{k+1}, b_0 halved. */
+static double seed_velocity_bound(const rpcf_params* prm, const double* v, const double* e, long stride, int nz,
+                                  int ny, int hnx) {
+    const double aphi = prm->n_pi_x / 2, beta = prm->n_pi_y / 2;
+    double su = 0.0, sv = 0.0, sw = 0.0;
+    double* b = (double*)calloc(2 * (size_t)nz + 4, sizeof(double));
+    for (int ky = 0; ky < ny; ky++)
+        for (int kx = 0; kx < hnx; kx++) {
+            if (kx == 0 && ky == 0) continue;
+            const size_t p = (size_t)ky * hnx + kx;
+            const double* a = v + 2 * (size_t)stride * p;
+            const double* h = e + 2 * (size_t)stride * p;
+            const double mult = kx == 0 ? 1.0 : 2.0;
+            const double al = (double)kx / aphi;
+            const double be = (ky >= ny / 2 + 1 ? (double)(ky - ny) : (double)ky) / beta;
+            const double k2 = al * al + be * be;
+            for (int k = 0; k < 2 * nz + 4; k++) b[k] = 0.0;
+            for (int k = nz - 2; k >= 0; k--) {
+                b[2 * k] = b[2 * (k + 2)] + 2.0 * (k + 1) * a[2 * (k + 1)];
+                b[2 * k + 1] = b[2 * (k + 2) + 1] + 2.0 * (k + 1) * a[2 * (k + 1) + 1];
+            }
+            b[0] = b[0] * 0.5;
+            b[1] = b[1] * 0.5;
+            for (int k = 0; k < nz; k++) {
+                const double ur = -(al * b[2 * k + 1] - be * h[2 * k + 1]) / k2;
+                const double ui = (al * b[2 * k] - be * h[2 * k]) / k2;
+                const double wr = -(be * b[2 * k + 1] + al * h[2 * k + 1]) / k2;
+                const double wi = (be * b[2 * k] + al * h[2 * k]) / k2;
+                su = su + mult * sqrt(ur * ur + ui * ui);
+                sv = sv + mult * sqrt(a[2 * k] * a[2 * k] + a[2 * k + 1] * a[2 * k + 1]);
+                sw = sw + mult * sqrt(wr * wr + wi * wi);
+            }
+        }
+    free(b);
+    double m = su > sv ? su : sv;
+    return m > sw ? m : sw;
+}
+
 int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int kmax, int degmax, double kexp,
                     double* v_hat, double* eta_hat, long stride) {
     int nx, ny, nz, pz, hnx;
@@ -1169,7 +1211,7 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
             if (kx == nx / 2 || ky == ny / 2) continue;
             if (kx > kmax || abs(kys) > kmax) continue;
             if (kx == 0 && kys < 0) continue;
-            double scale = amplitude * pow(1.0 + sqrt((double)kx * kx + (double)kys * kys), -kexp);
+            double scale = pow(1.0 + sqrt((double)kx * kx + (double)kys * kys), -kexp);
             size_t p = (size_t)ky * hnx + kx;
             for (int field = 0; field < 2; field++) {
                 for (int k = 0; k < nz; k++) g[k] = cmk(0.0, 0.0);
@@ -1195,6 +1237,16 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
                 a[2 * ((size_t)stride * pm + k) + 1] = -a[2 * ((size_t)stride * pp + k) + 1];
             }
         }
+    }
+    /* amplitude: the bound of the peak perturbation velocity */
+    const double bound = seed_velocity_bound(prm, v_hat, eta_hat, stride, nz, ny, hnx);
+    if (bound > 0.0) {
+        const double f = amplitude / bound;
+        for (size_t p = 0; p < P; p++)
+            for (int k = 0; k < 2 * nz; k++) {
+                v_hat[2 * (size_t)stride * p + k] *= f;
+                eta_hat[2 * (size_t)stride * p + k] *= f;
+            }
     }
     free(g);
     free(t);

@@ -1,6 +1,6 @@
 """Golden fixtures (tests/golden/tiny_couette.npz, written by tests/golden/make_golden.py
 from the oracle): the oracle reproduces them bit for bit; the HIP library matches them to
-the parity bars (1e-12 after one step, 1e-10 after ten)."""
+the parity bars (N to 1e-12, the state to 1e-12 after one step and 1e-10 after ten)."""
 import os
 import sys
 
@@ -39,6 +39,6 @@ def test_oracle_reproduces_golden():
 def test_gpu_matches_golden():
     g, s0, s1, h, s10 = run(rpcf.HIP_LIB)
     assert np.array_equal(s0[0], g["v0"]) and np.array_equal(s0[1], g["e0"])
-    assert rel(h[0], g["nv_hist"]) < 1e-13 and rel(h[1], g["ne_hist"]) < 1e-13
+    assert rel(h[0], g["nv_hist"]) < 1e-12 and rel(h[1], g["ne_hist"]) < 1e-12
     assert rel(s1[0], g["v1"]) < 1e-12 and rel(s1[1], g["e1"]) < 1e-12
     assert rel(s10[0], g["v10"]) < 1e-10 and rel(s10[1], g["e10"]) < 1e-10

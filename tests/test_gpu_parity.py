@@ -94,7 +94,10 @@ def test_velocity_and_nonlinear_match(name):
         assert np.array_equal(g.get_field(f), o.get_field(f)), f
     g.nonlinear(), o.nonlinear()
     a, b = g.get_nonlinear(), o.get_nonlinear()
-    assert rel(a[0], b[0]) < 1e-13 and rel(a[1], b[1]) < 1e-13
+    # the kernels sum the FFTs and the Chebyshev derivative in another order than the
+    # oracle (mixed-radix Stockham stages; a parity-class scan instead of the ascending
+    # sum of ddz_sm), so N agrees to round-off, not bit for bit
+    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
     g.close(), o.close()
 
 
@@ -148,3 +151,19 @@ def test_gpu_errors_are_reported():
             sim.bootstrap()
         with pytest.raises(rpcf.RpcfError, match="bad field"):
             sim.set_field(9, np.zeros((sim.s["P"], sim.s["nz"]), np.complex128))
+
+
+@pytest.mark.gpu
+def test_channel_workload_stays_bounded():
+    """the 128x129x128 channel workload (constant flow rate, seeded perturbation with peak
+    velocity bound 0.1) runs its 500 steps without blowing up: the state stays finite and
+    the perturbation stays of the order of its initial size"""
+    prm = params("C1")
+    with rpcf.Simulation(rpcf.HIP_LIB, prm) as sim:
+        v0, e0 = rpcf.seed_state(sim.dll, prm, 0, *IC["C1"])
+        sim.set_state(v0, e0)
+        sim.bootstrap()
+        sim.step(499)
+        v, e = sim.get_state()
+    assert np.isfinite(v).all() and np.isfinite(e).all()
+    assert np.abs(v[1:]).max() < 10 * np.abs(v0[1:]).max()

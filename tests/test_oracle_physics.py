@@ -122,7 +122,7 @@ def test_constant_flow_rate_holds():
     prm = params("TINY", u_profile=(1.0, 0.0, -1.0), flow_control=1, Ro=0.0)
     with rpcf.Simulation(rpcf.ORACLE_LIB, prm) as sim:
         s = sim.s
-        v0, e0 = rpcf.seed_state(sim.dll, prm, 11, 0.05, 4, 8, 0.0)
+        v0, e0 = rpcf.seed_state(sim.dll, prm, 11, 0.5, 4, 8, 0.0)
         sim.set_state(v0, e0)
         sim.bootstrap()
         sim.step(30)
