@@ -126,6 +126,60 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
     }
 };
 
+// cos(2 pi m / N), sin(2 pi m / N) for the composite radices (N = 12, 16)
+template <int N> struct TwTab;
+template <> struct TwTab<16> {
+    static constexpr double c[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                     0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                     -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                                     0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+    static constexpr double s[16] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                                     1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                     0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                     -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+};
+template <> struct TwTab<12> {
+    static constexpr double c[12] = {1.0, 0.86602540378443864676, 0.5, 0.0, -0.5, -0.86602540378443864676,
+                                     -1.0, -0.86602540378443864676, -0.5, 0.0, 0.5, 0.86602540378443864676};
+    static constexpr double s[12] = {0.0, 0.5, 0.86602540378443864676, 1.0, 0.86602540378443864676, 0.5,
+                                     0.0, -0.5, -0.86602540378443864676, -1.0, -0.86602540378443864676, -0.5};
+};
+
+// Radix R1*R2 butterfly in registers (Cooley-Tukey, n = R2 n1 + n2, k = k1 + R1 k2):
+// R2 DFTs of length R1, the twiddles exp(DIR 2 pi i n2 k1 / (R1 R2)), R1 DFTs of length R2.
+template <int R1, int R2, int DIR, typename C> struct DftComposite {
+    static __device__ __forceinline__ void run(C* v) {
+        using T = decltype(C().x);
+        constexpr int N = R1 * R2;
+        C y[R2][R1];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) {
+#pragma unroll
+            for (int n1 = 0; n1 < R1; n1++) y[n2][n1] = v[R2 * n1 + n2];
+            Dft<R1, DIR, C>::run(y[n2]);
+#pragma unroll
+            for (int k1 = 1; k1 < R1; k1++) {
+                const int m = (n2 * k1) % N;
+                if (m != 0) {
+                    const C w = cmake<C>((T)TwTab<N>::c[m], (T)(DIR * TwTab<N>::s[m]));
+                    y[n2][k1] = cmul(y[n2][k1], w);
+                }
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) {
+            C t[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; n2++) t[n2] = y[n2][k1];
+            Dft<R2, DIR, C>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; k2++) v[k1 + R1 * k2] = t[k2];
+        }
+    }
+};
+template <int DIR, typename C> struct Dft<16, DIR, C> : DftComposite<4, 4, DIR, C> {};
+template <int DIR, typename C> struct Dft<12, DIR, C> : DftComposite<4, 3, DIR, C> {};
+
 template <int R, int DIR, typename C>
 __device__ __forceinline__ void fft_stage(C* buf, int ld, int B, int n, int Ns, const C* __restrict__ tw, int tid,
                                           int nthr) {
@@ -218,6 +272,24 @@ constexpr int ct_rad(int n, int s) {
 // ceil(BT*N/R / kFftBlock) butterflies of a radix-R stage; BT = 0: RPCF_REGC/R.
 constexpr int kFftBlock = 256;
 
+// Two-pass plans for the compiled lengths (radix 16 and 12 butterflies held in registers,
+// one LDS exchange per pass); other lengths use ct_rad's radix-8 plan.
+constexpr int ct2_nst(int n) {
+    return (n == 48 || n == 64 || n == 96 || n == 128 || n == 192 || n == 256) ? 2 : (n == 384 ? 3 : 0);
+}
+constexpr int ct2_rad(int n, int s) {
+    return n == 48 ? (s == 0 ? 16 : 3)
+         : n == 64 ? (s == 0 ? 16 : 4)
+         : n == 96 ? (s == 0 ? 8 : 12)
+         : n == 128 ? (s == 0 ? 16 : 8)
+         : n == 192 ? (s == 0 ? 16 : 12)
+         : n == 256 ? 16
+         : n == 384 ? (s == 0 ? 16 : (s == 1 ? 12 : 2))
+         : 0;
+}
+constexpr int ct_plan_nst(int n) { return ct2_nst(n) ? ct2_nst(n) : ct_nst(n); }
+constexpr int ct_plan_rad(int n, int s) { return ct2_nst(n) ? ct2_rad(n, s) : ct_rad(n, s); }
+
 template <int R, int DIR, int N, int NS, int BT, typename C>
 __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __restrict__ tw, int tid, int nthr) {
     constexpr int nR = N / R;
@@ -259,20 +331,23 @@ __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __r
     __syncthreads();
 }
 
-template <int N, int DIR, int BT, typename C, int S = 0, int NS = 1>
+// TWO = 1: the two-pass plan where one exists; TWO = 0: the radix-8 plan (fewer registers)
+template <int N, int DIR, int BT, int TWO, typename C, int S = 0, int NS = 1>
 __device__ __forceinline__ void fft_ct(C* buf, int ld, int B, const C* tw, int tid, int nthr) {
-    if constexpr (S < ct_nst(N)) {
-        constexpr int R = ct_rad(N, S);
+    constexpr int nst = TWO ? ct_plan_nst(N) : ct_nst(N);
+    if constexpr (S < nst) {
+        constexpr int R = TWO ? ct_plan_rad(N, S) : ct_rad(N, S);
         fft_stage_ct<R, DIR, N, NS, BT>(buf, ld, B, tw, tid, nthr);
-        fft_ct<N, DIR, BT, C, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
+        fft_ct<N, DIR, BT, TWO, C, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
     }
 }
 
 // FN > 0: the compile-time plan of length FN (must equal pl.n); FN = 0: the runtime plan.
 // BT > 0 (compiled plans only): B <= BT sequences and nthr == kFftBlock.
-template <int FN, int DIR, int BT = 0, typename C>
+template <int FN, int DIR, int BT = 0, int TWO = 1, typename C>
 __device__ __forceinline__ void fft_any(C* buf, int ld, int B, const FftPlan& pl, int tid, int nthr) {
-    if constexpr (FN > 0) fft_ct<FN, DIR, BT>(buf, ld, B, (const C*)pl.tw, tid, nthr);
+    static_assert(FN == 0 || BT > 0, "compiled plans need the tile bound (their radix-12/16 stages size registers by it)");
+    if constexpr (FN > 0) fft_ct<FN, DIR, BT, TWO>(buf, ld, B, (const C*)pl.tw, tid, nthr);
     else fft_run<DIR>(buf, ld, B, pl, tid, nthr);
 }
 

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
             staged<8>(total, tid, nthr, load, [&](int i, double2 v) { ext(half, i, v); });
         }
         __syncthreads();
-        fft_any<FN, -1, kChebTile>(lds, ld, nb, pl, tid, nthr);
+        fft_any<FN, -1, kChebTile, 0>(lds, ld, nb, pl, tid, nthr);
         // even half: points q = m (PAD=0) or q = 2m (PAD=1), m = 0..N; odd half: q = 2m+1, m < N
         const int cnt = half ? N : N + 1;
         staged<8>(
