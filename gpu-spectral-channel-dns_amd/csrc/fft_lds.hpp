@@ -290,7 +290,17 @@ constexpr int ct2_rad(int n, int s) {
 constexpr int ct_plan_nst(int n) { return ct2_nst(n) ? ct2_nst(n) : ct_nst(n); }
 constexpr int ct_plan_rad(int n, int s) { return ct2_nst(n) ? ct2_rad(n, s) : ct_rad(n, s); }
 
-template <int R, int DIR, int N, int NS, int BT, typename C>
+// LDS address swizzle of a sequence element: a ^ ((a / Q) % Q) for Q = 8 or 16 permutes
+// the elements inside each aligned group of Q (a bijection on [0, n) when Q divides n).  The
+// first stage of a plan writes its butterfly outputs at stride R (j*R + r): unswizzled, the
+// lanes of a wave hit the same LDS banks; swizzled by Q = R they spread over all banks.  The
+// second stage reads through the same swizzle.
+template <int Q> __device__ __forceinline__ int swz(int a) {
+    if constexpr (Q == 0) return a;
+    else return a ^ ((a / Q) % Q);
+}
+
+template <int R, int DIR, int N, int NS, int BT, typename C, int QW = 0, int QR = 0>
 __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __restrict__ tw, int tid, int nthr) {
     constexpr int nR = N / R;
     constexpr int tstep = N / (NS * R);
@@ -305,7 +315,7 @@ __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __r
             const C* base = buf + b * ld;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                C x = base[j + r * nR];
+                C x = base[swz<QR>(j + r * nR)];
                 if (r > 0 && NS > 1) {
                     C w = tw[k * r * tstep];
                     if (DIR > 0) w.y = -w.y;
@@ -325,7 +335,7 @@ __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __r
             C* base = buf + b * ld;
             const int dst = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) base[dst + r * NS] = v[it][r];
+            for (int r = 0; r < R; r++) base[swz<QW>(dst + r * NS)] = v[it][r];
         }
     }
     __syncthreads();
@@ -337,7 +347,9 @@ __device__ __forceinline__ void fft_ct(C* buf, int ld, int B, const C* tw, int t
     constexpr int nst = TWO ? ct_plan_nst(N) : ct_nst(N);
     if constexpr (S < nst) {
         constexpr int R = TWO ? ct_plan_rad(N, S) : ct_rad(N, S);
-        fft_stage_ct<R, DIR, N, NS, BT>(buf, ld, B, tw, tid, nthr);
+        constexpr int R0 = TWO ? ct_plan_rad(N, 0) : ct_rad(N, 0);
+        constexpr int Q = (nst >= 2 && (R0 == 8 || R0 == 16)) ? R0 : 0;  // swizzle between stages 0 and 1
+        fft_stage_ct<R, DIR, N, NS, BT, C, S == 0 ? Q : 0, S == 1 ? Q : 0>(buf, ld, B, tw, tid, nthr);
         fft_ct<N, DIR, BT, TWO, C, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
     }
 }
