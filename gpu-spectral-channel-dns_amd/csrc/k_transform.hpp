@@ -499,6 +499,130 @@ __device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int
 }
 
 // -------------------------------------------------------------------------------------
+// Velocity recovery, getVelocityKernel_sm (source/velocity.cu:90-170), on tiles of
+// TB <= kChebTile pencils (kx fixed) with the Chebyshev derivatives from cheb_deriv_scan
+// (the reference sums each D_k directly, O(nz) per coefficient).  256 threads, member
+// blockIdx.z; thread-owned elements (pencil b = i / nz, k = i % nz) as in k_p2s_nl.
+// Pencil != (0,0):  u = (i a/k2) Dv - (i b/k2) eta,  w_span = (i b/k2) Dv + (i a/k2) eta,
+//                   ox = D w_span - i b v,  oy = -D u + i a v,  oz = eta;
+// pencil (0,0):     u = u0, w_span = w0, v = 0, ox = D w0, oy = -D u0, oz = 0.
+// -------------------------------------------------------------------------------------
+template <int FN>
+__global__ __launch_bounds__(256) void k_velocity_tile(const double2* __restrict__ sv, const double2* __restrict__ se,
+                                                       long sS, double2* __restrict__ F, long fstride, long sF, Dims d,
+                                                       Members mb, int TB, int ld) {
+    static_assert(FN > 0, "compiled lengths only");
+    RPCF_LDS(double2);
+    constexpr int N = FN / 2, nz = N + 1;
+    constexpr int KC = (kChebTile * nz + 255) / 256;
+    constexpr int CHM = (FN / 4 + 1 + 7) / 8;
+    const int slot = mb.ids[blockIdx.z];
+    const MemberParams& mp = mb.mp[slot];
+    sv += slot * sS;
+    se += slot * sS;
+    F += slot * sF;
+    const int kx = blockIdx.y, ky0 = blockIdx.x * TB;
+    const int nb = min(TB, d.ny - ky0);
+    const int tid = threadIdx.x, nthr = blockDim.x, total = nb * nz;
+    double2* fb = lds;
+    double2* tot = lds + TB * ld;
+    const long pstride = (long)d.hnx * nz;
+    const long pin = ((long)ky0 * d.hnx + kx) * nz;
+    double2 rv[KC], re[KC];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz;
+            rv[u] = sv[pin + b * pstride + k];
+            re[u] = se[pin + b * pstride + k];
+        }
+    }
+    auto mean = [&](int i) { return kx == 0 && ky0 + i / nz == 0; };
+    auto alpha_beta = [&](int b, double* ia, double* ib) {
+        const int ky = ky0 + b;
+        *ia = (double)kx / mp.aphi;
+        *ib = ky >= d.ny / 2 + 1 ? (double)(ky - d.ny) / mp.beta : (double)ky / mp.beta;
+    };
+    // D v
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz;
+            fb[b * ld + k] = rv[u];
+        }
+    }
+    __syncthreads();
+    cheb_deriv_scan<CHM>(fb, ld, nz, N, nb, tot, tid);
+    __syncthreads();
+    double2 su[KC], sw[KC];
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz;
+            if (mean(i)) {
+                su[u] = rv[u];
+                sw[u] = re[u];
+            } else {
+                double ia, ib;
+                alpha_beta(b, &ia, &ib);
+                const double kmn1 = 1.0 / (ia * ia + ib * ib);
+                const double2 t = fb[b * ld + k];
+                const double ca = ia * kmn1, cb = ib * kmn1;
+                // (i ca) t - (i cb) eta and (i cb) t + (i ca) eta
+                su[u] = make_double2((0.0 * t.x - ca * t.y) - (0.0 * re[u].x - cb * re[u].y),
+                                     (0.0 * t.y + ca * t.x) - (0.0 * re[u].y + cb * re[u].x));
+                sw[u] = make_double2((0.0 * t.x - cb * t.y) + (0.0 * re[u].x - ca * re[u].y),
+                                     (0.0 * t.y + cb * t.x) + (0.0 * re[u].y + ca * re[u].x));
+            }
+        }
+    }
+    __syncthreads();
+    // D u, D w_span
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz;
+            fb[b * ld + k] = su[u];
+            fb[b * ld + nz + k] = sw[u];
+        }
+    }
+    __syncthreads();
+    cheb_deriv_scan<CHM>(fb, ld, nz, N, nb, tot, tid);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz;
+            const double2 du = fb[b * ld + k], dw = fb[b * ld + nz + k];
+            double2 ox, oy, vv = rv[u], ee = re[u];
+            if (mean(i)) {
+                ox = dw;                               // D w0
+                oy = make_double2(-du.x, -du.y);       // -D u0
+                vv = ee = make_double2(0.0, 0.0);
+            } else {
+                double ia, ib;
+                alpha_beta(b, &ia, &ib);
+                const double nb_ = -1.0 * ib;
+                ox = make_double2(dw.x + (0.0 * vv.x - nb_ * vv.y), dw.y + (0.0 * vv.y + nb_ * vv.x));
+                oy = make_double2(du.x * -1.0 + (0.0 * vv.x - ia * vv.y), du.y * -1.0 + (0.0 * vv.y + ia * vv.x));
+            }
+            const long o = pin + b * pstride + k;
+            F[0 * fstride + o] = su[u];
+            F[1 * fstride + o] = sw[u];
+            F[2 * fstride + o] = vv;
+            F[3 * fstride + o] = ox;
+            F[4 * fstride + o] = oy;
+            F[5 * fstride + o] = ee;
+        }
+    }
+}
+
+// -------------------------------------------------------------------------------------
 // Forward Chebyshev transform of the three Lamb components, rhsNonlinearKernel
 // (source/nonlinear.cu:146-242) and the unpadded s2p of N_v and N_eta
 // (source/rhs.cpp:96-97).  Tile: TB <= kChebTile pencils (kx fixed), 256 threads; member

@@ -338,12 +338,23 @@ Members members(rpcf_ctx* c, const int* ids) {
 // ---- step pieces over a device slot list (asynchronous on c->stream) ---------------------
 int enq_velocity(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
     const Dims d = c->d;
-    const int thr = 64 * ((d.nz + 63) / 64);
+    const long fS = (long)d.P * d.nz;
     return launch(
         c, KID_VELOCITY,
         [&] {
-            k_velocity<<<dim3(d.P, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->F, (long)d.P * d.nz, c->sF, d,
-                                                            members(c, ids));
+            const int TB = c->TB_nl, ld = c->ld_zn;
+            dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
+            const size_t lds = sizeof(double2) * ((size_t)TB * ld + 256);
+            switch (2 * d.N) {
+#define RPCF_CASE(L) case L: k_velocity_tile<L><<<g, kFftThreads, lds, c->stream>>>(c->SV, c->SE, c->sS, c->F, fS, c->sF, d, members(c, ids), TB, ld); break;
+                RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                default: {
+                    const int thr = 64 * ((d.nz + 63) / 64);
+                    k_velocity<<<dim3(d.P, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->F, fS, c->sF, d,
+                                                                    members(c, ids));
+                }
+            }
         },
         cap);
 }
@@ -655,7 +666,8 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_span_inv<L, float>, lim);                 \
     allow_lds(k_span_fwd<L, float>, lim);                 \
     allow_lds(k_lamb_phys<L, float>, lim);                \
-    allow_lds(k_p2s_nl<L, float>, lim);
+    allow_lds(k_p2s_nl<L, float>, lim);                   \
+    allow_lds(k_velocity_tile<L>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     allow_lds(k_cheb_p2s, lim);

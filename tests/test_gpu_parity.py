@@ -90,8 +90,10 @@ def test_velocity_and_nonlinear_match(name):
     g, o = pair(prm)
     v0, e0 = rpcf.seed_state(o.dll, prm, 0, *IC[name])
     g.set_state(v0, e0), o.set_state(v0, e0)
+    # the kernels take the Chebyshev derivatives by a parity-class scan, the reference by the
+    # ascending sum of ddz_sm: the fields agree to round-off
     for f in range(6):
-        assert np.array_equal(g.get_field(f), o.get_field(f)), f
+        assert rel(g.get_field(f), o.get_field(f)) < 1e-13, f
     g.nonlinear(), o.nonlinear()
     a, b = g.get_nonlinear(), o.get_nonlinear()
     # the kernels sum the FFTs and the Chebyshev derivative in another order than the
