@@ -435,22 +435,56 @@ __global__ __launch_bounds__(256) void k_lin_tv_mfma(const double2* __restrict__
     }
 }
 
-// y_q[r] = sum_j M[r][j] x_q[j] for the two vectors of a slot, thread r = row r, j
-// ascending; M column-major, so a wave reads a contiguous column segment per j.
-__device__ __forceinline__ void gemv_rows2(const double2* __restrict__ M, const double2* x0, const double2* x1, int r,
-                                           int n, double2* y0, double2* y1) {
-    double a0r = 0, a0i = 0, a1r = 0, a1i = 0;
-#pragma unroll 8
-    for (int j = 0; j < n; j++) {
-        const double2 m = M[(long)j * n + r];
-        const double2 u0 = x0[j], u1 = x1[j];
-        a0r = fma(m.x, u0.x, fma(-m.y, u0.y, a0r));
-        a0i = fma(m.x, u0.y, fma(m.y, u0.x, a0i));
-        a1r = fma(m.x, u1.x, fma(-m.y, u1.y, a1r));
-        a1i = fma(m.x, u1.y, fma(m.y, u1.x, a1i));
+// y_q = M x_q for the two vectors of a slot with every thread of the block: G = nthr / n
+// groups of n threads (thread t = g*n + r owns row r), group g takes the columns
+// j = g, g + G, g + 2G, .. (fused multiply-adds, j ascending), so that the block reads the
+// column-major matrix (M[j*n + r] = M[r][j]) as one flat, coalesced stream; the G partial
+// sums meet in part[2][kSolveThreads] in group order.  M complex (double2) or real
+// (double).  The rows land in threads tid < n; every thread of the block calls it (it
+// synchronises).
+constexpr int kSolveThreads = 576;
+__device__ __forceinline__ void gemv_acc(double2 m, double2 u0, double2 u1, double* a) {
+    a[0] = fma(m.x, u0.x, fma(-m.y, u0.y, a[0]));
+    a[1] = fma(m.x, u0.y, fma(m.y, u0.x, a[1]));
+    a[2] = fma(m.x, u1.x, fma(-m.y, u1.y, a[2]));
+    a[3] = fma(m.x, u1.y, fma(m.y, u1.x, a[3]));
+}
+__device__ __forceinline__ void gemv_acc(double c, double2 u0, double2 u1, double* a) {
+    a[0] = fma(c, u0.x, a[0]);
+    a[1] = fma(c, u0.y, a[1]);
+    a[2] = fma(c, u1.x, a[2]);
+    a[3] = fma(c, u1.y, a[3]);
+}
+template <typename MT>
+__device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const double2* x0, const double2* x1, int n,
+                                             double2* part, int tid, int nthr, double2* y0, double2* y1) {
+    constexpr int B = 8;
+    const int G = nthr / n;
+    const int g = tid / n, r = tid - g * n;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g < G) {
+        int j = g;
+        for (; j + G * (B - 1) < n; j += G * B) {
+            MT m[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) m[u] = M[(long)(j + G * u) * n + r];
+#pragma unroll
+            for (int u = 0; u < B; u++) gemv_acc(m[u], x0[j + G * u], x1[j + G * u], a);
+        }
+        for (; j < n; j += G) gemv_acc(M[(long)j * n + r], x0[j], x1[j], a);
+        part[tid] = make_double2(a[0], a[1]);
+        part[kSolveThreads + tid] = make_double2(a[2], a[3]);
     }
-    *y0 = make_double2(a0r, a0i);
-    *y1 = make_double2(a1r, a1i);
+    __syncthreads();
+    if (tid < n) {
+        double2 s0 = part[tid], s1 = part[kSolveThreads + tid];
+        for (int q = 1; q < G; q++) {
+            s0 = zadd(s0, part[q * n + tid]);
+            s1 = zadd(s1, part[kSolveThreads + q * n + tid]);
+        }
+        *y0 = s0;
+        *y1 = s1;
+    }
 }
 
 // -------------------------------------------------------------------------------------
@@ -459,19 +493,20 @@ __device__ __forceinline__ void gemv_rows2(const double2* __restrict__ M, const 
 // source/rhs.cpp:205-215), v solve (solveEq, source/solver.cpp:107-123), T0 v^{n+1} (one
 // pass over T0), eta right-hand side (_get_linear_omega_y, source/linear.cpp:127-171), eta
 // solve.  Block 0 runs the mean flow of slot (0,0) (_get_linear_u0/_w0,
-// source/linear.cpp:173-205) and the constant-flow-rate correction.  Thread r = row r;
-// matrix slot blockIdx.x, member blockIdx.y.
+// source/linear.cpp:173-205) and the constant-flow-rate correction.  kSolveThreads
+// threads: thread r < nz assembles row r, the products (v solve, T0 v^{n+1}, eta solve) use every thread
+// (gemv_grouped); matrix slot blockIdx.x, member blockIdx.y.
 //   v rows r = 4..N, p = r-2:  (T4q v + k2 T2 v - k0 T0 v) + i (a2 T2 v + a0 T0 v)
 //   eta rows r = 2..N, p = r-1: (T2q eta + c0 T0 eta) + i ci T0 eta + i e T0 (v^n + v^{n+1})
 // -------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lin_solve(double2* __restrict__ SV, double2* __restrict__ SE, long sS,
+__global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restrict__ SV, double2* __restrict__ SE, long sS,
                                                    const double2* __restrict__ NL, long sNL,
                                                    const double2* __restrict__ MINV, long sM,
                                                    const double2* __restrict__ W, long sWl, Dims d, Members mb,
                                                    const double* __restrict__ T0t, const double* __restrict__ U,
                                                    const double* __restrict__ dU, const double* __restrict__ ddU, long sU,
                                                    const double* __restrict__ qw) {
-    __shared__ double2 rhs[2][256], vnew[2][256];
+    __shared__ double2 rhs[2][256], vnew[2][256], part[2][kSolveThreads];
     __shared__ double2 s_f;
     LinState st;
     LinParams lp;
@@ -528,8 +563,8 @@ __global__ __launch_bounds__(256) void k_lin_solve(double2* __restrict__ SV, dou
             }
             __syncthreads();
             double2 x = make_double2(0.0, 0.0), uh = make_double2(0.0, 0.0);
+            gemv_grouped(e ? Mo : Mv, rhs[0], rhs[1], n, &part[0][0], r, blockDim.x, &x, &uh);
             if (r < n) {
-                gemv_rows2(e ? Mo : Mv, rhs[0], rhs[1], r, n, &x, &uh);
                 vnew[0][r] = x;
                 vnew[1][r] = uh;
             }
@@ -581,23 +616,25 @@ __global__ __launch_bounds__(256) void k_lin_solve(double2* __restrict__ SV, dou
         rhs[1][r] = a1;
     }
     __syncthreads();
-    if (r < n) gemv_rows2(Mv, rhs[0], rhs[1], r, n, &vnew[0][r], &vnew[1][r]);
+    {
+        double2 y0, y1;
+        gemv_grouped(Mv, rhs[0], rhs[1], n, &part[0][0], r, blockDim.x, &y0, &y1);
+        if (r < n) vnew[0][r] = y0, vnew[1][r] = y1;
+    }
+    __syncthreads();
+    {
+        // T0 v^{n+1}, row p -> tv[.][p]
+        double2 y0, y1;
+        gemv_grouped(T0t, vnew[0], vnew[1], n, &part[0][0], r, blockDim.x, &y0, &y1);
+        __syncthreads();
+        if (r < n) part[0][r] = y0, part[1][r] = y1;
+    }
     __syncthreads();
     if (r < n) {
         double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
         if (r >= 2) {
             const int p = r - 1;
-            // T0 v^{n+1} at row p
-            double w0r = 0, w0i = 0, w1r = 0, w1i = 0;
-#pragma unroll 4
-            for (int j = 0; j < n; j++) {
-                const double c = T0t[(long)j * n + p];
-                const double2 x = vnew[0][j], y = vnew[1][j];
-                w0r = fma(c, x.x, w0r);
-                w0i = fma(c, x.y, w0i);
-                w1r = fma(c, y.x, w1r);
-                w1i = fma(c, y.y, w1i);
-            }
+            const double w0r = part[0][p].x, w0i = part[0][p].y, w1r = part[1][p].x, w1i = part[1][p].y;
             const double c0k = (1 - kmn * dt * 0.5 / Re), ci = -alpha * dt * 0.5 * U[p];
             const double e0 = -0.5 * beta0 * dU[p] * dt, e1 = -0.5 * beta1 * dU[p] * dt;
             const double2 s0 = W3[po0 + p], s2 = W4[po0 + p], sv0 = W0[po0 + p];
@@ -612,9 +649,9 @@ __global__ __launch_bounds__(256) void k_lin_solve(double2* __restrict__ SV, dou
         rhs[1][r] = a1;
     }
     __syncthreads();
+    double2 e0, e1;
+    gemv_grouped(Mo, rhs[0], rhs[1], n, &part[0][0], r, blockDim.x, &e0, &e1);
     if (r < n) {
-        double2 e0, e1;
-        gemv_rows2(Mo, rhs[0], rhs[1], r, n, &e0, &e1);
         st.se[po0 + r] = e0;
         st.sv[po0 + r] = vnew[0][r];
         if (two) {

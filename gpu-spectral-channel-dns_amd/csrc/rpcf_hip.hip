@@ -518,8 +518,7 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
     return launch(
         c, KID_LINEAR,
         [&] {
-            const int thr = 64 * ((d.nz + 63) / 64);
-            k_lin_solve<<<dim3(d.nslot, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
+            k_lin_solve<<<dim3(d.nslot, n), kSolveThreads, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
                                                                  c->WL, c->sWl, d, members(c, ids), c->T0t, c->U, c->DU,
                                                                  c->DDU, c->sU, c->qw);
         },
