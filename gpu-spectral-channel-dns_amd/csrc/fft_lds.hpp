@@ -37,11 +37,27 @@ template <typename T> struct cplx_t;
 template <> struct cplx_t<double> { using type = double2; };
 template <> struct cplx_t<float> { using type = float2; };
 
+// The complex helpers of the transforms allow multiply-add contraction (the library is
+// otherwise built with -ffp-contract=off): a product feeding a sum becomes one fused
+// multiply-add, which the FP64 butterflies are bound by.  The transforms then differ from
+// an uncontracted evaluation by round-off only.
 template <typename C> __device__ __forceinline__ C cmake(decltype(C().x) a, decltype(C().x) b) { C c; c.x = a; c.y = b; return c; }
-template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmake<C>(a.x + b.x, a.y + b.y); }
-template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmake<C>(a.x - b.x, a.y - b.y); }
-template <typename C> __device__ __forceinline__ C cmul(C a, C b) { return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C().x) s) { return cmake<C>(a.x * s, a.y * s); }
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) {
+#pragma clang fp contract(fast)
+    return cmake<C>(a.x + b.x, a.y + b.y);
+}
+template <typename C> __device__ __forceinline__ C csub(C a, C b) {
+#pragma clang fp contract(fast)
+    return cmake<C>(a.x - b.x, a.y - b.y);
+}
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+#pragma clang fp contract(fast)
+    return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C().x) s) {
+#pragma clang fp contract(fast)
+    return cmake<C>(a.x * s, a.y * s);
+}
 // multiply by i (DIR=+1) or -i (DIR=-1)
 template <int DIR, typename C> __device__ __forceinline__ C cjrot(C a) {
     return DIR > 0 ? cmake<C>(-a.y, a.x) : cmake<C>(a.y, -a.x);

@@ -777,3 +777,166 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const cplx<RI>* __restrict__ Z1,
         __syncthreads();
     }
 }
+
+// -------------------------------------------------------------------------------------
+// k_p2s_nl for the compiled lengths, with the work of a tile batched: the E and H sequences
+// of a Lamb component (both parity halves of its pz samples) go through one FFT call over
+// 2*TB sequences, as do N_v and N_eta at the end -- four FFT calls per tile instead of
+// eight, each with a full block of butterflies.  The samples of component g+1 are loaded
+// into registers while component g is transformed; the FFT and p2s twiddles are read from
+// LDS copies, so that no wait for a global load of theirs waits for those samples too.
+// Results are bit-identical to k_p2s_nl (same butterflies, same operation order).
+// LDS: [2*TB][ld] sequences, 256 scan totals, FN FFT twiddles, nz p2s twiddles.
+// -------------------------------------------------------------------------------------
+template <int FN, typename RI = double>
+__global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1, long zstride, long sY,
+                                                 double2* __restrict__ NL, long sNL, Dims d, FftPlan plN,
+                                                 const double2* __restrict__ tw4, int TB, int ld, Members mb) {
+    static_assert(FN > 0, "compiled lengths only");
+    RPCF_LDS(double2);
+    constexpr int N = FN / 2, nz = N + 1, L = FN, pz = FN + 1;
+    constexpr int KC = (kChebTile * nz + 255) / 256;
+    constexpr int KL = (kChebTile * pz + 255) / 256;
+    constexpr int CHM = (FN / 4 + 1 + 7) / 8;
+    const int mslot = mb.ids[blockIdx.z];
+    const int ph = mb.phase[mslot];
+    const double aphi = mb.mp[mslot].aphi, beta = mb.mp[mslot].beta;
+    const long PS = (long)d.P * nz;
+    Z1 += mslot * sY;
+    double2* NLv = NL + mslot * sNL + ph * PS;        // N_v, current
+    double2* NLe = NL + mslot * sNL + (2 + ph) * PS;  // N_eta, current
+    const int kx = blockIdx.y, ky0 = blockIdx.x * TB;
+    const int nb = min(TB, d.ny - ky0), ny = d.ny;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    double2* fb = lds;
+    double2* tot = lds + 2 * TB * ld;
+    double2* twl = tot + 256;
+    double2* tw4l = twl + FN;
+    {
+        const double2* twg = (const double2*)plN.tw;
+        for (int i = tid; i < FN; i += nthr) twl[i] = twg[i];
+        for (int i = tid; i < nz; i += nthr) tw4l[i] = tw4[i];
+    }
+    FftPlan pll = plN;
+    pll.tw = twl;
+    const double factor = (1.0 / (d.pz - 1));
+    const int total = nb * nz, nsamp = nb * pz;
+    const cplx<RI>* in0 = Z1 + (long)kx * d.pz * ny + ky0;
+    double2 smp[KL];
+    auto load = [&](int g) {
+        const cplx<RI>* in = in0 + g * zstride;
+#pragma unroll
+        for (int u = 0; u < KL; u++) {
+            const int i = tid + u * nthr;
+            if (i < nsamp) {
+                const int row = i / nb, b = i - row * nb;
+                smp[u] = to_d2(in[(long)row * ny + b]);
+            }
+        }
+    };
+    load(0);
+    double2 ck[3][KC];
+    auto component = [&](auto gc) {
+        constexpr int g = decltype(gc)::value;
+        // even rows f_2m -> E of pencil b (sequence b), odd rows f_2m+1 -> H (sequence nb+b)
+#pragma unroll
+        for (int u = 0; u < KL; u++) {
+            const int i = tid + u * nthr;
+            if (i < nsamp) {
+                const int row = i / nb, b = i - row * nb, m = row >> 1;
+                const double2 v = smp[u];
+                if ((row & 1) == 0) {
+                    double2* sq = fb + b * ld;
+                    sq[m] = v;
+                    if (m >= 1 && m <= N - 1) sq[L - m] = v;
+                } else {
+                    double2* sq = fb + (nb + b) * ld;
+                    sq[m] = v;
+                    sq[L - 1 - m] = v;
+                }
+            }
+        }
+        if constexpr (g < 2) load(g + 1);
+        __syncthreads();
+        fft_any<FN, -1, 2 * kChebTile>(fb, ld, 2 * nb, pll, tid, nthr);
+#pragma unroll
+        for (int u = 0; u < KC; u++) {
+            const int i = tid + u * nthr;
+            if (i < total) {
+                const int b = i / nz, k = i - b * nz;
+                const double2 e = fb[b * ld + k], h = fb[(nb + b) * ld + k];
+                double2 t = cscale(cadd(e, cmul(tw4l[k], h)), factor);
+                if (k == 0) t = cscale(t, 0.5);
+                ck[g][u] = t;
+            }
+        }
+        __syncthreads();
+    };
+    component(IntC<0>{});
+    component(IntC<1>{});
+    component(IntC<2>{});
+    // L_x, L_y -> LDS [b][0..1][nz] for the derivative scan
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz;
+            fb[b * ld + k] = ck[0][u];
+            fb[b * ld + nz + k] = ck[1][u];
+        }
+    }
+    __syncthreads();
+    cheb_deriv_scan<CHM>(fb, ld, nz, N, nb, tot, tid);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz, ky = ky0 + b;
+            const double2 lx = ck[0][u], ly = ck[1][u], lz = ck[2][u];
+            double2 nv, ne;
+            if (kx == 0 && ky == 0) {
+                nv = lx;
+                ne = ly;
+            } else {
+                const double ia = (double)kx / aphi;
+                const double ib = ky >= ny / 2 + 1 ? (double)(ky - ny) / beta : (double)ky / beta;
+                const double kmn = ia * ia + ib * ib;
+                const double2 dx = fb[b * ld + k], dy = fb[b * ld + nz + k];
+                double ure = kmn * lz.x - ia * dx.y;
+                double uim = kmn * lz.y + ia * dx.x;
+                ure = ure - ib * dy.y;
+                uim = uim + ib * dy.x;
+                const double wre = -1.0 * ia * ly.y + ib * lx.y;
+                const double wim = ia * ly.x - ib * lx.x;
+                nv = make_double2(ure, uim);
+                ne = make_double2(wre, wim);
+            }
+            ck[0][u] = nv;
+            ck[1][u] = ne;
+        }
+    }
+    __syncthreads();
+    // unpadded s2p of N_v (sequences b) and N_eta (sequences nb+b) in one FFT call
+#pragma unroll
+    for (int u = 0; u < KC; u++) {
+        const int i = tid + u * nthr;
+        if (i < total) {
+            const int b = i / nz, k = i - b * nz;
+            s2p_even_ext(fb + b * ld, ck[0][u], k, N);
+            s2p_even_ext(fb + (nb + b) * ld, ck[1][u], k, N);
+        }
+    }
+    __syncthreads();
+    fft_any<FN, -1, 2 * kChebTile>(fb, ld, 2 * nb, pll, tid, nthr);
+    staged<8>(
+        2 * total, tid, nthr,
+        [&](int i) {
+            const int h = i >= total, j = i - h * total, b = j / nz, p = j - b * nz;
+            return fb[(h * nb + b) * ld + p];
+        },
+        [&](int i, double2 v) {
+            const int h = i >= total, j = i - h * total, b = j / nz, p = j - b * nz;
+            (h ? NLe : NLv)[((long)(ky0 + b) * d.hnx + kx) * nz + p] = v;
+        });
+}

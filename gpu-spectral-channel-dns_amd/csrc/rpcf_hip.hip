@@ -62,8 +62,9 @@ struct rpcf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int lds_limit = 65536;
+    int ncu = 256;                   // compute units of the device
     // tiles: sequences per block and LDS row stride (elements) of each transform kernel
-    int TB_s2p = 1, TB_nl = 1, TK_inv = 1, TK_fwd = 1, RP = 1;
+    int TB_s2p = 1, TB_nl = 1, TB_nl2 = 0, TK_inv = 1, TK_fwd = 1, RP = 1;
     int ld_zn = 0, ld_my = 0, ld_mx = 0, zld = 0;
     int mixed = 0;                   // 1: transform intermediates and x/span FFTs in FP32
     FftPlan pl_x{}, pl_y{}, pl_zp{}, pl_zn{};
@@ -461,6 +462,21 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     }
     // AB2 history swap (source/rhs.cpp:107-108): the new N lands in the old "previous" buffers
     k_flip<<<1, 64, 0, c->stream>>>(c->PH, ids, n);
+    if (c->TB_nl2) {
+        const int TB = c->TB_nl2, ld = c->ld_zn;
+        dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
+        const size_t lds = sizeof(double2) * (2 * (size_t)TB * ld + 256 + 2 * d.N + d.nz);
+        return launch(
+            c, KID_P2S_NL,
+            [&] {
+                switch (2 * d.N) {
+#define RPCF_CASE(L) case L: k_p2s_nl2<L, R><<<g, kFftThreads, lds, c->stream>>>(Y, yS, c->sY, c->NL, c->sNL, d, c->pl_zn, tw4, TB, ld, mb); break;
+                    RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                }
+            },
+            cap);
+    }
     {
         const int TB = c->TB_nl, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
@@ -559,6 +575,9 @@ int init_ctx(rpcf_ctx* c) {
     int lim = 0;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lim > 0)
         c->lds_limit = lim;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
+        c->ncu = ncu;
 
     const SlotSizes ss = slot_sizes(d);
     c->sF = ss.F; c->sS = ss.S; c->sNL = ss.NL; c->sY = ss.Y; c->sW = ss.W; c->sZ = ss.Z;
@@ -642,6 +661,15 @@ int init_ctx(rpcf_ctx* c) {
     c->TB_nl = pick_tile(kChebTile, 2 * d.N, c->cap_zn, lim3, [&](int t) {
         return sizeof(double2) * ((size_t)t * c->ld_zn + 256);
     });
+    // the batched k_p2s_nl2 (compiled lengths), two resident blocks per CU; 0: not used
+    c->TB_nl2 = 0;
+#define RPCF_CASE(L)                                                                                     \
+    if (2 * d.N == L)                                                                                    \
+        c->TB_nl2 = pick_tile(kChebTile, 1, kChebTile, lim / 2, [&](int t) {                                   \
+            return sizeof(double2) * (2 * (size_t)t * c->ld_zn + 256 + 2 * d.N + d.nz);                 \
+        });
+    RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
     c->TK_inv = pick_tile(kSpanTile, d.my, c->cap_y, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_my; });
     c->TK_fwd = c->TK_inv;
     c->RP = pick_tile(kLambPairs, 6 * d.mx, c->cap_x, lim3, [&](int t) { return sizeof(double2) * 6 * (size_t)t * c->ld_mx; });
@@ -666,6 +694,8 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_span_fwd<L, float>, lim);                 \
     allow_lds(k_lamb_phys<L, float>, lim);                \
     allow_lds(k_p2s_nl<L, float>, lim);                   \
+    allow_lds(k_p2s_nl2<L>, lim);                         \
+    allow_lds(k_p2s_nl2<L, float>, lim);                  \
     allow_lds(k_velocity_tile<L>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
