@@ -311,23 +311,25 @@ constexpr int ct_plan_rad(int n, int s) { return ct2_nst(n) ? ct2_rad(n, s) : ct
 // first stage of a plan writes its butterfly outputs at stride R (j*R + r): unswizzled, the
 // lanes of a wave hit the same LDS banks; swizzled by Q = R they spread over all banks.  The
 // second stage reads through the same swizzle.
-template <int Q> __device__ __forceinline__ int swz(int a) {
+template <int Q> __device__ __forceinline__ unsigned swz(unsigned a) {
     if constexpr (Q == 0) return a;
     else return a ^ ((a / Q) % Q);
 }
 
 template <int R, int DIR, int N, int NS, int BT, typename C, int QW = 0, int QR = 0>
 __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __restrict__ tw, int tid, int nthr) {
-    constexpr int nR = N / R;
-    constexpr int tstep = N / (NS * R);
+    // unsigned index arithmetic: divisions by the compile-time strides become shifts and
+    // multiplies without sign fix-ups
+    constexpr unsigned nR = N / R;
+    constexpr unsigned tstep = N / (NS * R);
     constexpr int MAXPT = BT ? (BT * nR + kFftBlock - 1) / kFftBlock : RPCF_REGC / R;
-    const int tot = B * nR;
+    const unsigned tot = B * nR;
     C v[MAXPT][R];
 #pragma unroll
     for (int it = 0; it < MAXPT; it++) {
-        const int t = tid + it * nthr;
+        const unsigned t = tid + it * nthr;
         if (t < tot) {
-            const int b = t / nR, j = t - b * nR, k = j % NS;
+            const unsigned b = t / nR, j = t - b * nR, k = j % NS;
             const C* base = buf + b * ld;
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -345,11 +347,11 @@ __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __r
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < MAXPT; it++) {
-        const int t = tid + it * nthr;
+        const unsigned t = tid + it * nthr;
         if (t < tot) {
-            const int b = t / nR, j = t - b * nR, k = j % NS;
+            const unsigned b = t / nR, j = t - b * nR, k = j % NS;
             C* base = buf + b * ld;
-            const int dst = (j / NS) * NS * R + k;
+            const unsigned dst = (j / NS) * NS * R + k;
 #pragma unroll
             for (int r = 0; r < R; r++) base[swz<QW>(dst + r * NS)] = v[it][r];
         }

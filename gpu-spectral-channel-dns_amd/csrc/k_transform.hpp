@@ -100,61 +100,71 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
     const long slot = ids[blockIdx.z / 6];
     F += slot * sF;
     Y += slot * sY;
-    const int nb = min(TB, d.ny - ky0);
-    const int N = FN ? FN / 2 : d.N, nz = N + 1, nout = PAD ? 2 * N + 1 : nz;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const long pstride = (long)d.hnx * nz;
-    const double2* in = F + f * fstride + ((long)ky0 * d.hnx + kx) * nz;
-    cplx<RO>* out = Y + f * ystride + (long)kx * nout * d.ny + ky0;
-    const int total = nb * nz;
-    auto load = [&](int i) {
-        const int b = i / nz, k = i - b * nz;
-        return in[b * pstride + k];
-    };
-    auto ext = [&](int half, int i, double2 a) {
-        const int b = i / nz, k = i - b * nz;
-        if (half == 0) s2p_even_ext(lds + b * ld, a, k, N);
-        else s2p_odd_ext(lds + b * ld, a, k, N, tw4);
-    };
-    // with a compiled length and TB <= kChebTile the tile's coefficients stay in registers
-    // for both halves
-    constexpr int KC = FN ? (kChebTile * (FN / 2 + 1) + 255) / 256 : 1;
-    const bool inreg = FN > 0 && total <= KC * nthr;
-    double2 a[KC];
-    if (inreg) {
-#pragma unroll
-        for (int u = 0; u < KC; u++) {
-            const int i = tid + u * nthr;
-            if (i < total) a[u] = load(i);
-        }
-    }
-#pragma unroll
-    for (int half = 0; half < (PAD ? 2 : 1); half++) {
+    const int nb_rt = min(TB, d.ny - ky0);
+    // A full tile of a compiled length fixes the tile width, nz and the LDS stride (2N + 2),
+    // so the index arithmetic reduces to constant multiplies and shifts; offsets inside a
+    // slot's field fit 32 bits.
+    auto body = [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        const int nb = NB ? NB : nb_rt;
+        const int N = FN ? FN / 2 : d.N, nz = N + 1, nout = PAD ? 2 * N + 1 : nz;
+        const int ldl = FN ? FN + 2 : ld;
+        const int tid = threadIdx.x, nthr = blockDim.x, ny = d.ny;
+        const int pstride = d.hnx * nz;
+        const double2* in = F + (f * fstride + (ky0 * d.hnx + kx) * nz);
+        cplx<RO>* out = Y + (f * ystride + kx * nout * ny + ky0);
+        const int total = nb * nz;
+        auto load = [&](int i) {
+            const unsigned b = (unsigned)i / nz, k = i - b * nz;
+            return in[b * pstride + k];
+        };
+        auto ext = [&](int half, int i, double2 a) {
+            const unsigned b = (unsigned)i / nz, k = i - b * nz;
+            if (half == 0) s2p_even_ext(lds + b * ldl, a, k, N);
+            else s2p_odd_ext(lds + b * ldl, a, k, N, tw4);
+        };
+        // with a compiled length and TB <= kChebTile the tile's coefficients stay in
+        // registers for both halves
+        constexpr int KC = FN ? (kChebTile * (FN / 2 + 1) + 255) / 256 : 1;
+        const bool inreg = FN > 0 && total <= KC * nthr;
+        double2 a[KC];
         if (inreg) {
 #pragma unroll
             for (int u = 0; u < KC; u++) {
                 const int i = tid + u * nthr;
-                if (i < total) ext(half, i, a[u]);
+                if (i < total) a[u] = load(i);
             }
-        } else {
-            staged<8>(total, tid, nthr, load, [&](int i, double2 v) { ext(half, i, v); });
         }
-        __syncthreads();
-        fft_any<FN, -1, kChebTile, 0>(lds, ld, nb, pl, tid, nthr);
-        // even half: points q = m (PAD=0) or q = 2m (PAD=1), m = 0..N; odd half: q = 2m+1, m < N
-        const int cnt = half ? N : N + 1;
-        staged<8>(
-            cnt * nb, tid, nthr,
-            [&](int i) {
-                const int m = i / nb, b = i - m * nb;
-                return lds[b * ld + m];
-            },
-            [&](int i, double2 v) {
-                const int m = i / nb, b = i - m * nb;
-                out[(long)(PAD ? 2 * m + half : m) * d.ny + b] = to_c<cplx<RO>>(v);
-            });
-        __syncthreads();
-    }
+#pragma unroll
+        for (int half = 0; half < (PAD ? 2 : 1); half++) {
+            if (inreg) {
+#pragma unroll
+                for (int u = 0; u < KC; u++) {
+                    const int i = tid + u * nthr;
+                    if (i < total) ext(half, i, a[u]);
+                }
+            } else {
+                staged<8>(total, tid, nthr, load, [&](int i, double2 v) { ext(half, i, v); });
+            }
+            __syncthreads();
+            fft_any<FN, -1, kChebTile, 0>(lds, ldl, nb, pl, tid, nthr);
+            // even half: points q = m (PAD=0) or q = 2m (PAD=1), m = 0..N; odd half: q = 2m+1
+            const int cnt = half ? N : N + 1;
+            staged<8>(
+                cnt * nb, tid, nthr,
+                [&](int i) {
+                    const unsigned m = (unsigned)i / nb, b = i - m * nb;
+                    return lds[b * ldl + m];
+                },
+                [&](int i, double2 v) {
+                    const unsigned m = (unsigned)i / nb, b = i - m * nb;
+                    out[(PAD ? 2 * m + half : m) * ny + b] = to_c<cplx<RO>>(v);
+                });
+            __syncthreads();
+        }
+    };
+    if (FN > 0 && nb_rt == kChebTile) body(IntC<kChebTile>{});
+    else body(IntC<0>{});
 }
 
 // -------------------------------------------------------------------------------------
@@ -174,33 +184,42 @@ __global__ __launch_bounds__(256) void k_span_inv(const cplx<R>* __restrict__ Y,
     const long slot = ids[blockIdx.z / 6];
     Y += slot * sY;
     W += slot * sW;
-    const int nk = min(TK, d.nxh - kx0);
-    const int my = FN ? FN : d.my, tid = threadIdx.x, nthr = blockDim.x;
-    const C2* in = Y + f * ystride;
-    staged<12>(
-        nk * my, tid, nthr,
-        [&](int i) {
-            const int kxi = i / my, s = i - kxi * my;
-            const int ky = ky_of_span(d, s);
-            return ky >= 0 ? in[((long)(kx0 + kxi) * planes + q) * d.ny + ky] : cmake<C2>(0, 0);
-        },
-        [&](int i, C2 v) {
-            const int kxi = i / my, s = i - kxi * my;
-            lds[kxi * ld + s] = v;
-        });
-    __syncthreads();
-    fft_any<FN, +1, kSpanTile>(lds, ld, nk, pl, tid, nthr);
-    C2* out = W + f * wstride;
-    staged<12>(
-        my * nk, tid, nthr,
-        [&](int i) {
-            const int s = i / nk, kxi = i - s * nk;
-            return lds[kxi * ld + s];
-        },
-        [&](int i, C2 v) {
-            const int s = i / nk, kxi = i - s * nk;
-            out[((long)q * my + s) * d.nxh + kx0 + kxi] = v;
-        });
+    const int nk_rt = min(TK, d.nxh - kx0);
+    // a full tile of a compiled length: constant tile width, length and LDS stride (my + 1);
+    // offsets inside a slot's field fit 32 bits
+    auto body = [&](auto nkc) {
+        constexpr int NK = decltype(nkc)::value;
+        const int nk = NK ? NK : nk_rt;
+        const int my = FN ? FN : d.my, ldl = FN ? FN + 1 : ld, tid = threadIdx.x, nthr = blockDim.x;
+        const int ny = d.ny, nxh = d.nxh;
+        const C2* in = Y + f * ystride;
+        staged<12>(
+            nk * my, tid, nthr,
+            [&](int i) {
+                const unsigned kxi = (unsigned)i / my, s = i - kxi * my;
+                const int ky = ky_of_span(d, s);
+                return ky >= 0 ? in[((kx0 + kxi) * planes + q) * ny + ky] : cmake<C2>(0, 0);
+            },
+            [&](int i, C2 v) {
+                const unsigned kxi = (unsigned)i / my, s = i - kxi * my;
+                lds[kxi * ldl + s] = v;
+            });
+        __syncthreads();
+        fft_any<FN, +1, kSpanTile>(lds, ldl, nk, pl, tid, nthr);
+        C2* out = W + f * wstride;
+        staged<12>(
+            my * nk, tid, nthr,
+            [&](int i) {
+                const unsigned s = (unsigned)i / nk, kxi = i - s * nk;
+                return lds[kxi * ldl + s];
+            },
+            [&](int i, C2 v) {
+                const unsigned s = (unsigned)i / nk, kxi = i - s * nk;
+                out[(q * my + s) * nxh + kx0 + kxi] = v;
+            });
+    };
+    if (FN > 0 && nk_rt == kSpanTile) body(IntC<kSpanTile>{});
+    else body(IntC<0>{});
 }
 
 // Load two half-spectrum rows (kx < nxh) as one packed length-mx spectrum
@@ -251,76 +270,88 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
     W += slot * sW;
     Z2 += slot * sZ;
     meanU += slot * sPz;
-    const int nrp = min(RP, d.my / 2 - rp0);
-    const int mx = FN ? FN : d.mx, my = d.my, nxh = d.nxh, tid = threadIdx.x, nthr = blockDim.x;
-    // packed spectra Z = X_A + i X_B of the row pairs (c2r_pack, element by element): each
-    // kept coefficient pair (A_k, B_k), k < nxh, is loaded once and gives Z_k and
-    // Z_{mx-k}; Z_k = 0 for nxh <= k <= mx - nxh
-    constexpr int KL = 3;  // loads per thread in flight: 6 * kLambPairs * 64 / 256
-    for (int base = tid; base < 6 * nrp * nxh; base += KL * nthr) {
-        C2 va[KL], vb[KL];
+    const int nrp_rt = min(RP, d.my / 2 - rp0);
+    // The compiled lengths fix mx, nxh = (mx/3*2)/2, hnx = nxh + 1 and the LDS stride
+    // mx + 1; a full tile fixes nrp = kLambPairs -- the index arithmetic then reduces to
+    // constant multiplies and shifts.  Offsets inside a slot fit 32 bits.
+    auto body = [&](auto nc) {
+        constexpr int NC = decltype(nc)::value;
+        const int nrp = NC ? NC : nrp_rt;
+        const int mx = FN ? FN : d.mx, nxh = FN ? (FN / 3 * 2) / 2 : d.nxh, hnx = nxh + 1;
+        const int lds_ld = FN ? FN + 1 : ld;
+        const int my = d.my, tid = threadIdx.x, nthr = blockDim.x;
+        const int ws = (int)wstride, zs = (int)zstride;
+        // packed spectra Z = X_A + i X_B of the row pairs (c2r_pack, element by element):
+        // each kept coefficient pair (A_k, B_k), k < nxh, is loaded once and gives Z_k and
+        // Z_{mx-k}; Z_k = 0 for nxh <= k <= mx - nxh
+        constexpr int KL = 3;  // loads per thread in flight: 6 * kLambPairs * 64 / 256
+        for (int base = tid; base < 6 * nrp * nxh; base += KL * nthr) {
+            C2 va[KL], vb[KL];
 #pragma unroll
-        for (int u = 0; u < KL; u++) {
-            const int i = base + u * nthr;
-            if (i < 6 * nrp * nxh) {
-                const int seq = i / nxh, k = i - seq * nxh;
-                const int f = seq / nrp, ip = seq - f * nrp;
-                const C2* rA = W + f * wstride + ((long)q * my + 2 * (rp0 + ip)) * nxh;
-                va[u] = rA[k];
-                vb[u] = rA[nxh + k];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < KL; u++) {
-            const int i = base + u * nthr;
-            if (i < 6 * nrp * nxh) {
-                const int seq = i / nxh, k = i - seq * nxh;
-                C2 a = va[u], b = vb[u];
-                C2* z = lds + seq * ld;
-                if (k == 0) {
-                    a.y = 0;
-                    b.y = 0;
-                } else {
-                    z[mx - k] = cmake<C2>(a.x + b.y, -a.y + b.x);
+            for (int u = 0; u < KL; u++) {
+                const int i = base + u * nthr;
+                if (i < 6 * nrp * nxh) {
+                    const int seq = i / nxh, k = i - seq * nxh;
+                    const int f = seq / nrp, ip = seq - f * nrp;
+                    const C2* rA = W + (f * ws + (q * my + 2 * (rp0 + ip)) * nxh);
+                    va[u] = rA[k];
+                    vb[u] = rA[nxh + k];
                 }
-                z[k] = cmake<C2>(a.x - b.y, a.y + b.x);
+            }
+#pragma unroll
+            for (int u = 0; u < KL; u++) {
+                const int i = base + u * nthr;
+                if (i < 6 * nrp * nxh) {
+                    const int seq = i / nxh, k = i - seq * nxh;
+                    C2 a = va[u], b = vb[u];
+                    C2* z = lds + seq * lds_ld;
+                    if (k == 0) {
+                        a.y = 0;
+                        b.y = 0;
+                    } else {
+                        z[mx - k] = cmake<C2>(a.x + b.y, -a.y + b.x);
+                    }
+                    z[k] = cmake<C2>(a.x - b.y, a.y + b.x);
+                }
             }
         }
-    }
-    for (int i = tid; i < 6 * nrp * (mx - 2 * nxh + 1); i += nthr) {
-        const int seq = i / (mx - 2 * nxh + 1), k = i - seq * (mx - 2 * nxh + 1);
-        lds[seq * ld + nxh + k] = cmake<C2>(0, 0);
-    }
-    __syncthreads();
-    fft_any<FN, +1, 6 * kLambPairs>(lds, ld, 6 * nrp, pl, tid, nthr);
-    // Ro U(y) is uniform in x and span, so it reaches only the (0,0) pencil of L_z, which
-    // rhsNonlinear does not use: the FP32 form leaves it out (in FP32 its round-off would
-    // swamp the perturbation products); the FP64 form keeps the reference's expression.
-    const R mU = sizeof(R) == 8 ? (R)meanU[q] : R(0);
-    for (int i = tid; i < nrp * mx; i += nthr) {
-        const int ip = i / mx, x = i - ip * mx;
-        const C2 u = lds[(0 * nrp + ip) * ld + x], v = lds[(1 * nrp + ip) * ld + x];
-        const C2 w = lds[(2 * nrp + ip) * ld + x], ox = lds[(3 * nrp + ip) * ld + x];
-        const C2 oy = lds[(4 * nrp + ip) * ld + x], oz = lds[(5 * nrp + ip) * ld + x];
-        const R lxa = oz.x * v.x - oy.x * w.x - Ro * w.x, lxb = oz.y * v.y - oy.y * w.y - Ro * w.y;
-        const R lya = ox.x * w.x - oz.x * u.x, lyb = ox.y * w.y - oz.y * u.y;
-        const R lza = oy.x * u.x - ox.x * v.x + Ro * (u.x + mU), lzb = oy.y * u.y - ox.y * v.y + Ro * (u.y + mU);
-        lds[(0 * nrp + ip) * ld + x] = cmake<C2>(lxa, lxb);
-        lds[(1 * nrp + ip) * ld + x] = cmake<C2>(lya, lyb);
-        lds[(2 * nrp + ip) * ld + x] = cmake<C2>(lza, lzb);
-    }
-    __syncthreads();
-    fft_any<FN, -1, 3 * kLambPairs>(lds, ld, 3 * nrp, pl, tid, nthr);
-    const int hnx = d.hnx;
-    for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
-        const int seq = i / hnx, k = i - seq * hnx;
-        const int g = seq / nrp, ip = seq - g * nrp;
-        C2 xa, xb;
-        r2c_unpack(lds + seq * ld, k, mx, (R)fac, &xa, &xb);
-        C2* o = Z2 + g * zstride + ((long)q * my + 2 * (rp0 + ip)) * zld;
-        o[k] = xa;
-        o[zld + k] = xb;
-    }
+        const int nzero = mx - 2 * nxh + 1;
+        for (int i = tid; i < 6 * nrp * nzero; i += nthr) {
+            const int seq = i / nzero, k = i - seq * nzero;
+            lds[seq * lds_ld + nxh + k] = cmake<C2>(0, 0);
+        }
+        __syncthreads();
+        fft_any<FN, +1, 6 * kLambPairs>(lds, lds_ld, 6 * nrp, pl, tid, nthr);
+        // Ro U(y) is uniform in x and span, so it reaches only the (0,0) pencil of L_z, which
+        // rhsNonlinear does not use: the FP32 form leaves it out (in FP32 its round-off would
+        // swamp the perturbation products); the FP64 form keeps the reference's expression.
+        const R mU = sizeof(R) == 8 ? (R)meanU[q] : R(0);
+        for (int i = tid; i < nrp * mx; i += nthr) {
+            const int ip = i / mx, x = i - ip * mx;
+            const C2 u = lds[(0 * nrp + ip) * lds_ld + x], v = lds[(1 * nrp + ip) * lds_ld + x];
+            const C2 w = lds[(2 * nrp + ip) * lds_ld + x], ox = lds[(3 * nrp + ip) * lds_ld + x];
+            const C2 oy = lds[(4 * nrp + ip) * lds_ld + x], oz = lds[(5 * nrp + ip) * lds_ld + x];
+            const R lxa = oz.x * v.x - oy.x * w.x - Ro * w.x, lxb = oz.y * v.y - oy.y * w.y - Ro * w.y;
+            const R lya = ox.x * w.x - oz.x * u.x, lyb = ox.y * w.y - oz.y * u.y;
+            const R lza = oy.x * u.x - ox.x * v.x + Ro * (u.x + mU), lzb = oy.y * u.y - ox.y * v.y + Ro * (u.y + mU);
+            lds[(0 * nrp + ip) * lds_ld + x] = cmake<C2>(lxa, lxb);
+            lds[(1 * nrp + ip) * lds_ld + x] = cmake<C2>(lya, lyb);
+            lds[(2 * nrp + ip) * lds_ld + x] = cmake<C2>(lza, lzb);
+        }
+        __syncthreads();
+        fft_any<FN, -1, 3 * kLambPairs>(lds, lds_ld, 3 * nrp, pl, tid, nthr);
+        for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
+            const int seq = i / hnx, k = i - seq * hnx;
+            const int g = seq / nrp, ip = seq - g * nrp;
+            C2 xa, xb;
+            r2c_unpack(lds + seq * lds_ld, k, mx, (R)fac, &xa, &xb);
+            C2* o = Z2 + (g * zs + (q * my + 2 * (rp0 + ip)) * zld);
+            o[k] = xa;
+            o[zld + k] = xb;
+        }
+    };
+    if (FN > 0 && nrp_rt == kLambPairs) body(IntC<kLambPairs>{});
+    else body(IntC<0>{});
 }
 
 // x C2R of one field (no Lamb): W[q][s][nxh] -> phys[q][s][mx] (rpcf_spec_to_phys).
@@ -386,32 +417,40 @@ __global__ __launch_bounds__(256) void k_span_fwd(const cplx<R>* __restrict__ Z2
     const long slot = ids[blockIdx.z / 3];
     Z2 += slot * sZ;
     Z1 += slot * sY;
-    const int nk = min(TK, d.hnx - kx0);
-    const int my = FN ? FN : d.my, ny = d.ny, tid = threadIdx.x, nthr = blockDim.x;
-    const C2* in = Z2 + g * zstride;
-    staged<12>(
-        my * nk, tid, nthr,
-        [&](int i) {
-            const int s = i / nk, kxi = i - s * nk;
-            return in[((long)q * my + s) * zld + kx0 + kxi];
-        },
-        [&](int i, C2 v) {
-            const int s = i / nk, kxi = i - s * nk;
-            lds[kxi * ld + s] = v;
-        });
-    __syncthreads();
-    fft_any<FN, -1, kSpanTile>(lds, ld, nk, pl, tid, nthr);
-    C2* out = Z1 + g * ostride;
-    staged<8>(
-        nk * ny, tid, nthr,
-        [&](int i) {
-            const int kxi = i / ny, ky = i - kxi * ny;
-            return lds[kxi * ld + span_of(d, ky)];
-        },
-        [&](int i, C2 v) {
-            const int kxi = i / ny, ky = i - kxi * ny;
-            out[((long)(kx0 + kxi) * planes + q) * ny + ky] = v;
-        });
+    const int nk_rt = min(TK, d.hnx - kx0);
+    // a full tile of a compiled length: constant tile width, length and LDS stride (my + 1);
+    // offsets inside a slot's field fit 32 bits
+    auto body = [&](auto nkc) {
+        constexpr int NK = decltype(nkc)::value;
+        const int nk = NK ? NK : nk_rt;
+        const int my = FN ? FN : d.my, ldl = FN ? FN + 1 : ld, ny = d.ny, tid = threadIdx.x, nthr = blockDim.x;
+        const C2* in = Z2 + g * zstride;
+        staged<12>(
+            my * nk, tid, nthr,
+            [&](int i) {
+                const unsigned s = (unsigned)i / nk, kxi = i - s * nk;
+                return in[(q * my + s) * zld + kx0 + kxi];
+            },
+            [&](int i, C2 v) {
+                const unsigned s = (unsigned)i / nk, kxi = i - s * nk;
+                lds[kxi * ldl + s] = v;
+            });
+        __syncthreads();
+        fft_any<FN, -1, kSpanTile>(lds, ldl, nk, pl, tid, nthr);
+        C2* out = Z1 + g * ostride;
+        staged<8>(
+            nk * ny, tid, nthr,
+            [&](int i) {
+                const unsigned kxi = (unsigned)i / ny, ky = i - kxi * ny;
+                return lds[kxi * ldl + span_of(d, ky)];
+            },
+            [&](int i, C2 v) {
+                const unsigned kxi = (unsigned)i / ny, ky = i - kxi * ny;
+                out[((kx0 + kxi) * planes + q) * ny + ky] = v;
+            });
+    };
+    if (FN > 0 && nk_rt == kSpanTile) body(IntC<kSpanTile>{});
+    else body(IntC<0>{});
 }
 
 // Load one field's wall-normal values (P points, P = pz or nz) of TB pencils as the
