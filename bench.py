@@ -135,10 +135,11 @@ def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64):
 
 
 def ensemble_leg(n, nsteps):
-    """n independent simulations of mixed grid sizes (FP64): one group per shape, the
-    groups concurrently, against the same members run one at a time"""
+    """n independent simulations of mixed grid sizes in the mixed form (FP32 transforms,
+    FP64 solves): one group per shape, the groups concurrently, against the same members
+    run one at a time"""
     from rpcf import ensemble
-    ms = ensemble.mixed_members(n, seed=0)
+    ms = ensemble.mixed_members(n, seed=0, precision=1)
     wall, runs = ensemble.run_grouped(ms, nsteps)
     alone_total, alone_each, _ = ensemble.run_alone(ms, nsteps)
     per100_group = [r.seconds / nsteps * 100 for r in runs for _ in r.members]
@@ -146,7 +147,7 @@ def ensemble_leg(n, nsteps):
     gp = sum(ensemble.grid_points(m) for m in ms) * nsteps
     log(f"ensemble {n} members x {nsteps} steps: grouped {wall:.3f}s, one at a time {alone_total:.3f}s")
     return {
-        "members": n, "steps": nsteps, "dtype": "f64",
+        "members": n, "steps": nsteps, "dtype": "f32 transforms / f64 solves",
         "groups": {f"{r.shape[0]}x{r.shape[1]}x{r.shape[2]}": len(r.members) for r in runs},
         "group_s": {f"{r.shape[0]}x{r.shape[1]}x{r.shape[2]}": round(r.seconds, 4) for r in runs},
         "one_at_a_time_s": {f"{r.shape[0]}x{r.shape[1]}x{r.shape[2]}": round(sum(alone_each[i] for i in r.members), 4)

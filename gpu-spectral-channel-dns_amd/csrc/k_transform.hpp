@@ -22,6 +22,9 @@
 // Pencils per block of the Chebyshev kernels: 8 x 16 B = one 128-byte line per point row
 // of the [kx][q][ky] layouts.  Columns per block of the span kernels.
 constexpr int kChebTile = 8;
+// Pencils per block of k_cheb_s2p for a compiled 2N: shorter transforms take wider tiles,
+// so that a block keeps about the same work (and LDS) as at 2N = 256.
+constexpr int s2p_tile_max(int fn) { return fn >= 192 || fn == 0 ? kChebTile : fn >= 96 ? 2 * kChebTile : 4 * kChebTile; }
 constexpr int kSpanTile = 16;
 constexpr int kLambPairs = 2;  // pairs of span rows per block of k_lamb_phys
 
@@ -123,9 +126,10 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
             if (half == 0) s2p_even_ext(lds + b * ldl, a, k, N);
             else s2p_odd_ext(lds + b * ldl, a, k, N, tw4);
         };
-        // with a compiled length and TB <= kChebTile the tile's coefficients stay in
+        // with a compiled length and TB <= s2p_tile_max(FN) the tile's coefficients stay in
         // registers for both halves
-        constexpr int KC = FN ? (kChebTile * (FN / 2 + 1) + 255) / 256 : 1;
+        constexpr int TBX = s2p_tile_max(FN);
+        constexpr int KC = FN ? (TBX * (FN / 2 + 1) + 255) / 256 : 1;
         const bool inreg = FN > 0 && total <= KC * nthr;
         double2 a[KC];
         if (inreg) {
@@ -147,7 +151,7 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
                 staged<8>(total, tid, nthr, load, [&](int i, double2 v) { ext(half, i, v); });
             }
             __syncthreads();
-            fft_any<FN, -1, kChebTile, 0>(lds, ldl, nb, pl, tid, nthr);
+            fft_any<FN, -1, TBX, 0>(lds, ldl, nb, pl, tid, nthr);
             // even half: points q = m (PAD=0) or q = 2m (PAD=1), m = 0..N; odd half: q = 2m+1
             const int cnt = half ? N : N + 1;
             staged<8>(
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
             __syncthreads();
         }
     };
-    if (FN > 0 && nb_rt == kChebTile) body(IntC<kChebTile>{});
+    if (FN > 0 && nb_rt == s2p_tile_max(FN)) body(IntC<s2p_tile_max(FN)>{});
     else body(IntC<0>{});
 }
 

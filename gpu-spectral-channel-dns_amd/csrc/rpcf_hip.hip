@@ -534,7 +534,9 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
     return launch(
         c, KID_LINEAR,
         [&] {
-            k_lin_solve<<<dim3(d.nslot, n), kSolveThreads, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
+            // about four column groups of nz threads (all of kSolveThreads at nz = 129)
+            const int thr = std::min(kSolveThreads, 64 * ((4 * d.nz + 63) / 64));
+            k_lin_solve<<<dim3(d.nslot, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
                                                                  c->WL, c->sWl, d, members(c, ids), c->T0t, c->U, c->DU,
                                                                  c->DDU, c->sU, c->qw);
         },
@@ -657,7 +659,13 @@ int init_ctx(rpcf_ctx* c) {
     c->ld_zn = 2 * d.N + 2;  // >= 2N (FFT) and >= 2 nz (L_x, L_y of k_p2s_nl)
     c->ld_my = d.my + 1;
     c->ld_mx = d.mx + 1;
-    c->TB_s2p = pick_tile(kChebTile, 2 * d.N, c->cap_zn, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_zn; });
+    {
+        int tmax = kChebTile;  // the runtime-plan kernel sizes its registers for kChebTile
+#define RPCF_CASE(L) if (2 * d.N == L) tmax = s2p_tile_max(L);
+        RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+        c->TB_s2p = pick_tile(tmax, 2 * d.N, c->cap_zn, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_zn; });
+    }
     c->TB_nl = pick_tile(kChebTile, 2 * d.N, c->cap_zn, lim3, [&](int t) {
         return sizeof(double2) * ((size_t)t * c->ld_zn + 256);
     });

@@ -37,10 +37,11 @@ class Member:
         return (self.prm.mx, self.prm.my, self.prm.mz)
 
 
-def mixed_members(n=64, seed=0, dt=0.002):
+def mixed_members(n=64, seed=0, dt=0.002, precision=1):
     """the mixed-size ensemble: member i has seed i; grid uniform over ENSEMBLE_GRIDS,
     Re ~ U[500, 3000], Ro ~ U[0, 0.5], plane Couette (U = y) or pressure-driven channel
-    (U = 1 - y^2, constant flow rate) by a fair coin; Lx = 2 pi, Lz = pi"""
+    (U = 1 - y^2, constant flow rate) by a fair coin; Lx = 2 pi, Lz = pi; precision 1 (the
+    mixed form: FP32 transforms, FP64 solves) or 0 (FP64)"""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -50,7 +51,7 @@ def mixed_members(n=64, seed=0, dt=0.002):
         couette = bool(rng.integers(2))
         prm = B.make_params(**grid_shape(*g), n_pi_x=2.0, n_pi_y=1.0, Re=re, Ro=ro, dt=dt,
                             u_profile=(0.0, 1.0, 0.0) if couette else (1.0, 0.0, -1.0),
-                            flow_control=0 if couette else 1)
+                            flow_control=0 if couette else 1, precision=precision)
         out.append(Member(prm, seed=i, tag=f"{g[0]}x{g[1]}x{g[2]} {'couette' if couette else 'channel'}"))
     return out
 
