@@ -27,6 +27,10 @@ constexpr int kChebTile = 8;
 constexpr int s2p_tile_max(int fn) { return fn >= 192 || fn == 0 ? kChebTile : fn >= 96 ? 2 * kChebTile : 4 * kChebTile; }
 constexpr int kSpanTile = 16;
 constexpr int kLambPairs = 2;  // pairs of span rows per block of k_lamb_phys
+// ... and their bounds for a compiled transform length: shorter rows and columns take wider
+// tiles, so that a block keeps about the work and LDS it has at length 192.
+constexpr int span_tile_max(int fn) { return fn >= 192 || fn == 0 ? kSpanTile : 2 * kSpanTile; }
+constexpr int lamb_pairs_max(int fn) { return fn >= 192 || fn == 0 ? kLambPairs : fn >= 96 ? 2 * kLambPairs : 4 * kLambPairs; }
 
 template <int V> struct IntC { static constexpr int value = V; };
 
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(256) void k_span_inv(const cplx<R>* __restrict__ Y,
                 lds[kxi * ldl + s] = v;
             });
         __syncthreads();
-        fft_any<FN, +1, kSpanTile>(lds, ldl, nk, pl, tid, nthr);
+        fft_any<FN, +1, span_tile_max(FN)>(lds, ldl, nk, pl, tid, nthr);
         C2* out = W + f * wstride;
         staged<12>(
             my * nk, tid, nthr,
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(256) void k_span_inv(const cplx<R>* __restrict__ Y,
                 out[(q * my + s) * nxh + kx0 + kxi] = v;
             });
     };
-    if (FN > 0 && nk_rt == kSpanTile) body(IntC<kSpanTile>{});
+    if (FN > 0 && nk_rt == span_tile_max(FN)) body(IntC<span_tile_max(FN)>{});
     else body(IntC<0>{});
 }
 
@@ -276,7 +280,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
     meanU += slot * sPz;
     const int nrp_rt = min(RP, d.my / 2 - rp0);
     // The compiled lengths fix mx, nxh = (mx/3*2)/2, hnx = nxh + 1 and the LDS stride
-    // mx + 1; a full tile fixes nrp = kLambPairs -- the index arithmetic then reduces to
+    // mx + 1; a full tile fixes nrp = lamb_pairs_max(mx) -- the index arithmetic then reduces to
     // constant multiplies and shifts.  Offsets inside a slot fit 32 bits.
     auto body = [&](auto nc) {
         constexpr int NC = decltype(nc)::value;
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
         // packed spectra Z = X_A + i X_B of the row pairs (c2r_pack, element by element):
         // each kept coefficient pair (A_k, B_k), k < nxh, is loaded once and gives Z_k and
         // Z_{mx-k}; Z_k = 0 for nxh <= k <= mx - nxh
-        constexpr int KL = 3;  // loads per thread in flight: 6 * kLambPairs * 64 / 256
+        constexpr int KL = 3;  // loads per thread in flight (6 * 2 * 64 / 256 at mx = 192)
         for (int base = tid; base < 6 * nrp * nxh; base += KL * nthr) {
             C2 va[KL], vb[KL];
 #pragma unroll
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
             lds[seq * lds_ld + nxh + k] = cmake<C2>(0, 0);
         }
         __syncthreads();
-        fft_any<FN, +1, 6 * kLambPairs>(lds, lds_ld, 6 * nrp, pl, tid, nthr);
+        fft_any<FN, +1, 6 * lamb_pairs_max(FN)>(lds, lds_ld, 6 * nrp, pl, tid, nthr);
         // Ro U(y) is uniform in x and span, so it reaches only the (0,0) pencil of L_z, which
         // rhsNonlinear does not use: the FP32 form leaves it out (in FP32 its round-off would
         // swamp the perturbation products); the FP64 form keeps the reference's expression.
@@ -343,7 +347,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
             lds[(2 * nrp + ip) * lds_ld + x] = cmake<C2>(lza, lzb);
         }
         __syncthreads();
-        fft_any<FN, -1, 3 * kLambPairs>(lds, lds_ld, 3 * nrp, pl, tid, nthr);
+        fft_any<FN, -1, 3 * lamb_pairs_max(FN)>(lds, lds_ld, 3 * nrp, pl, tid, nthr);
         for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
             const int seq = i / hnx, k = i - seq * hnx;
             const int g = seq / nrp, ip = seq - g * nrp;
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
             o[zld + k] = xb;
         }
     };
-    if (FN > 0 && nrp_rt == kLambPairs) body(IntC<kLambPairs>{});
+    if (FN > 0 && nrp_rt == lamb_pairs_max(FN)) body(IntC<lamb_pairs_max(FN)>{});
     else body(IntC<0>{});
 }
 
@@ -440,7 +444,7 @@ __global__ __launch_bounds__(256) void k_span_fwd(const cplx<R>* __restrict__ Z2
                 lds[kxi * ldl + s] = v;
             });
         __syncthreads();
-        fft_any<FN, -1, kSpanTile>(lds, ldl, nk, pl, tid, nthr);
+        fft_any<FN, -1, span_tile_max(FN)>(lds, ldl, nk, pl, tid, nthr);
         C2* out = Z1 + g * ostride;
         staged<8>(
             nk * ny, tid, nthr,
@@ -453,7 +457,7 @@ __global__ __launch_bounds__(256) void k_span_fwd(const cplx<R>* __restrict__ Z2
                 out[((kx0 + kxi) * planes + q) * ny + ky] = v;
             });
     };
-    if (FN > 0 && nk_rt == kSpanTile) body(IntC<kSpanTile>{});
+    if (FN > 0 && nk_rt == span_tile_max(FN)) body(IntC<span_tile_max(FN)>{});
     else body(IntC<0>{});
 }
 

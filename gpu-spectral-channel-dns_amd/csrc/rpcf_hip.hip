@@ -678,9 +678,15 @@ int init_ctx(rpcf_ctx* c) {
         });
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-    c->TK_inv = pick_tile(kSpanTile, d.my, c->cap_y, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_my; });
+    int tk_max = kSpanTile, rp_max = kLambPairs;  // the runtime-plan kernels size registers for these
+#define RPCF_CASE(L)                          \
+    if (d.my == L) tk_max = span_tile_max(L); \
+    if (d.mx == L) rp_max = lamb_pairs_max(L);
+    RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+    c->TK_inv = pick_tile(tk_max, d.my, c->cap_y, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_my; });
     c->TK_fwd = c->TK_inv;
-    c->RP = pick_tile(kLambPairs, 6 * d.mx, c->cap_x, lim3, [&](int t) { return sizeof(double2) * 6 * (size_t)t * c->ld_mx; });
+    c->RP = pick_tile(rp_max, 6 * d.mx, c->cap_x, lim3, [&](int t) { return sizeof(double2) * 6 * (size_t)t * c->ld_mx; });
     if (!c->TB_s2p || !c->TB_nl || !c->TK_inv || !c->RP)
         return fail(c, RPCF_EINVAL, "grid too large for the LDS tiles (limit %d bytes)", lim);
     allow_lds(k_cheb_s2p<1>, lim);
