@@ -24,6 +24,9 @@
 constexpr int kChebTile = 8;
 // Pencils per block of k_cheb_s2p for a compiled 2N: shorter transforms take wider tiles,
 // so that a block keeps about the same work (and LDS) as at 2N = 256.
+// Pencils per block of the kernels with the derivative scan (k_velocity_tile, k_p2s_nl2):
+// 16 (scan with 4 chunks per class) for 2N <= 128, else 8.
+constexpr int cheb_nl_tile_max(int fn) { return fn > 0 && fn <= 128 ? 2 * kChebTile : kChebTile; }
 constexpr int s2p_tile_max(int fn) { return fn >= 192 || fn == 0 ? kChebTile : fn >= 96 ? 2 * kChebTile : 4 * kChebTile; }
 constexpr int kSpanTile = 16;
 constexpr int kLambPairs = 2;  // pairs of span rows per block of k_lamb_phys
@@ -496,15 +499,18 @@ __global__ __launch_bounds__(256) void k_cheb_p2s(const double2* __restrict__ Z1
 
 // Chebyshev derivative of ddz_sm (source/RPCFKernels.cu:476-520), D_k = sum_{j = k+1,
 // k+3, .. <= N} 2j a_j with D_0 halved, of the two coefficient arrays A_c = fb + b*ld + c*nz
-// (c = 0, 1) of each of the nb <= 8 pencils of a tile, in place.  D is a suffix sum over
-// each parity class of j: two-level scan with 8 chunks per (pencil, array, class) on 256
-// threads, written over the coefficients of the other class (read before the barrier).
-// Callers synchronise before (arrays written) and after (D read).  tot: 256 LDS entries.
-template <int CHM>  // chunk length bound: >= ceil((N/2 + 1) / 8)
+// (c = 0, 1) of each of the nb <= 1024 / (4 NCH) pencils of a tile, in place.  D is a suffix
+// sum over each parity class of j: two-level scan with NCH chunks per (pencil, array,
+// class) on 256 threads (NCH = 8: 8 pencils, NCH = 4: 16 pencils), written over the
+// coefficients of the other class (read before the barrier).  Callers synchronise before
+// (arrays written) and after (D read).  tot: 256 LDS entries.
+template <int CHM, int NCH = 8>  // CHM: chunk length bound, >= ceil((N/2 + 1) / NCH)
 __device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int N, int nb, double2* tot, int tid) {
-    const int b = tid >> 5, c2 = (tid >> 4) & 1, par = (tid >> 3) & 1, c = tid & 7;
+    static_assert(NCH == 4 || NCH == 8, "4 or 8 chunks per parity class");
+    constexpr int SH = NCH == 8 ? 3 : 2;
+    const int b = tid >> (SH + 2), c2 = (tid >> (SH + 1)) & 1, par = (tid >> SH) & 1, c = tid & (NCH - 1);
     const int Lc = (N - par) / 2 + 1;  // class size: j = par, par+2, .. <= N
-    const int CH = (Lc + 7) / 8;
+    const int CH = (Lc + NCH - 1) / NCH;
     const int t0 = c * CH, t1 = min(t0 + CH, Lc);
     double2* A = fb + b * ld + c2 * nz;
     double2 cj[CHM];
@@ -526,8 +532,8 @@ __device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int
     __syncthreads();
     if (b < nb) {
         double rre = 0.0, rim = 0.0;
-        for (int cc = 7; cc > c; cc--) {
-            const double2 o = tot[(tid & ~7) | cc];
+        for (int cc = NCH - 1; cc > c; cc--) {
+            const double2 o = tot[(tid & ~(NCH - 1)) | cc];
             rre += o.x;
             rim += o.y;
         }
@@ -561,8 +567,9 @@ __global__ __launch_bounds__(256) void k_velocity_tile(const double2* __restrict
     static_assert(FN > 0, "compiled lengths only");
     RPCF_LDS(double2);
     constexpr int N = FN / 2, nz = N + 1;
-    constexpr int KC = (kChebTile * nz + 255) / 256;
-    constexpr int CHM = (FN / 4 + 1 + 7) / 8;
+    constexpr int TBC = cheb_nl_tile_max(FN), NCH = TBC > kChebTile ? 4 : 8;
+    constexpr int KC = (TBC * nz + 255) / 256;
+    constexpr int CHM = (FN / 4 + 1 + NCH - 1) / NCH;
     const int slot = mb.ids[blockIdx.z];
     const MemberParams& mp = mb.mp[slot];
     sv += slot * sS;
@@ -601,7 +608,7 @@ __global__ __launch_bounds__(256) void k_velocity_tile(const double2* __restrict
         }
     }
     __syncthreads();
-    cheb_deriv_scan<CHM>(fb, ld, nz, N, nb, tot, tid);
+    cheb_deriv_scan<CHM, NCH>(fb, ld, nz, N, nb, tot, tid);
     __syncthreads();
     double2 su[KC], sw[KC];
 #pragma unroll
@@ -638,7 +645,7 @@ __global__ __launch_bounds__(256) void k_velocity_tile(const double2* __restrict
         }
     }
     __syncthreads();
-    cheb_deriv_scan<CHM>(fb, ld, nz, N, nb, tot, tid);
+    cheb_deriv_scan<CHM, NCH>(fb, ld, nz, N, nb, tot, tid);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < KC; u++) {
@@ -842,9 +849,10 @@ __global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1
     static_assert(FN > 0, "compiled lengths only");
     RPCF_LDS(double2);
     constexpr int N = FN / 2, nz = N + 1, L = FN, pz = FN + 1;
-    constexpr int KC = (kChebTile * nz + 255) / 256;
-    constexpr int KL = (kChebTile * pz + 255) / 256;
-    constexpr int CHM = (FN / 4 + 1 + 7) / 8;
+    constexpr int TBC = cheb_nl_tile_max(FN), NCH = TBC > kChebTile ? 4 : 8;
+    constexpr int KC = (TBC * nz + 255) / 256;
+    constexpr int KL = (TBC * pz + 255) / 256;
+    constexpr int CHM = (FN / 4 + 1 + NCH - 1) / NCH;
     const int mslot = mb.ids[blockIdx.z];
     const int ph = mb.phase[mslot];
     const double aphi = mb.mp[mslot].aphi, beta = mb.mp[mslot].beta;
@@ -905,7 +913,7 @@ __global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1
         }
         if constexpr (g < 2) load(g + 1);
         __syncthreads();
-        fft_any<FN, -1, 2 * kChebTile>(fb, ld, 2 * nb, pll, tid, nthr);
+        fft_any<FN, -1, 2 * TBC>(fb, ld, 2 * nb, pll, tid, nthr);
 #pragma unroll
         for (int u = 0; u < KC; u++) {
             const int i = tid + u * nthr;
@@ -933,7 +941,7 @@ __global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1
         }
     }
     __syncthreads();
-    cheb_deriv_scan<CHM>(fb, ld, nz, N, nb, tot, tid);
+    cheb_deriv_scan<CHM, NCH>(fb, ld, nz, N, nb, tot, tid);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < KC; u++) {
@@ -975,7 +983,7 @@ __global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1
         }
     }
     __syncthreads();
-    fft_any<FN, -1, 2 * kChebTile>(fb, ld, 2 * nb, pll, tid, nthr);
+    fft_any<FN, -1, 2 * TBC>(fb, ld, 2 * nb, pll, tid, nthr);
     staged<8>(
         2 * total, tid, nthr,
         [&](int i) {

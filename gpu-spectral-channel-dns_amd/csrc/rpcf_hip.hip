@@ -64,7 +64,7 @@ struct rpcf_ctx {
     int lds_limit = 65536;
     int ncu = 256;                   // compute units of the device
     // tiles: sequences per block and LDS row stride (elements) of each transform kernel
-    int TB_s2p = 1, TB_nl = 1, TB_nl2 = 0, TK_inv = 1, TK_fwd = 1, RP = 1;
+    int TB_s2p = 1, TB_nl = 1, TB_nl2 = 0, TB_vel = 1, TK_inv = 1, TK_fwd = 1, RP = 1;
     int ld_zn = 0, ld_my = 0, ld_mx = 0, zld = 0;
     int mixed = 0;                   // 1: transform intermediates and x/span FFTs in FP32
     FftPlan pl_x{}, pl_y{}, pl_zp{}, pl_zn{};
@@ -343,7 +343,7 @@ int enq_velocity(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
     return launch(
         c, KID_VELOCITY,
         [&] {
-            const int TB = c->TB_nl, ld = c->ld_zn;
+            const int TB = c->TB_vel, ld = c->ld_zn;
             dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
             const size_t lds = sizeof(double2) * ((size_t)TB * ld + 256);
             switch (2 * d.N) {
@@ -673,8 +673,17 @@ int init_ctx(rpcf_ctx* c) {
     c->TB_nl2 = 0;
 #define RPCF_CASE(L)                                                                                     \
     if (2 * d.N == L)                                                                                    \
-        c->TB_nl2 = pick_tile(kChebTile, 1, kChebTile, lim / 2, [&](int t) {                                   \
+        c->TB_nl2 = pick_tile(cheb_nl_tile_max(L), 1, cheb_nl_tile_max(L), lim / 2, [&](int t) {             \
             return sizeof(double2) * (2 * (size_t)t * c->ld_zn + 256 + 2 * d.N + d.nz);                 \
+        });
+    RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+    // k_velocity_tile (compiled lengths): the tile of the scan kernels
+    c->TB_vel = c->TB_nl;
+#define RPCF_CASE(L)                                                                              \
+    if (2 * d.N == L)                                                                             \
+        c->TB_vel = pick_tile(cheb_nl_tile_max(L), 1, cheb_nl_tile_max(L), lim3, [&](int t) {     \
+            return sizeof(double2) * ((size_t)t * c->ld_zn + 256);                                \
         });
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
