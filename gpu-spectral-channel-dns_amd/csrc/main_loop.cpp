@@ -10,18 +10,25 @@
 //   output_prefix / recovery_prefix
 // and optionally two extension lines (absent: reference behaviour):
 //   c0 c1 c2 flow_control precision    base flow U = c0 + c1 y + c2 y^2, flow-rate control, 0 FP64 / 1 mixed
-//   seed amplitude kmax degmax kexp    seeded initial field (rpcf_seed_state)
+//   seed amplitude kmax degmax kexp    seeded initial field (rpcf_seed_state); without
+//                                      this line start_type 0 takes the reference's
+//                                      analytic field (init_flow_kernel)
 //
 // --lib selects the library implementing the ABI (default: librpcf_hip.so beside this
 // executable); the same driver runs the CPU oracle for testing.
 //
+// Outputs at step 0 and every save_internal steps: the reference's velocity and vorticity
+// files (<prefix>_velocity.<step>.dat, <prefix>_vortricity.<step>.dat, physical fields on
+// the padded grid), the max |field| lines of its statistics(), and a spectral state
+// snapshot <prefix>_state.<step>.bin (an addition).  Output prefix "-": no files.
+//
 // Differences from the reference, by design: errors are reported and end the program
-// with a non-zero status instead of exit()/system("pause"); the initial field is the
-// seeded one (the reference's analytic init_flow_kernel is not reproduced); snapshots
-// are spectral state files; the recovery file is portable (sizes in the header, pencil
-// stride nz) and lives at <recovery_prefix>.rcv.
+// with a non-zero status instead of exit()/system("pause"); the recovery file is portable
+// (sizes in the header, pencil stride nz) and lives at <recovery_prefix>.rcv.
 #include <dlfcn.h>
 
+#include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -50,6 +57,9 @@ struct Api {
     int (*step)(rpcf_ctx*, int);
     int (*bootstrap)(rpcf_ctx*);
     int (*seed_state)(const rpcf_params*, uint64_t, double, int, int, double, double*, double*, long);
+    int (*get_field)(rpcf_ctx*, int, double*, long);
+    int (*phys_to_spec)(rpcf_ctx*, const double*, double*, long, int);
+    int (*spec_to_phys)(rpcf_ctx*, const double*, double*, long, int);
 };
 
 template <typename F>
@@ -70,13 +80,15 @@ bool load_api(const std::string& path, Api* a) {
            sym(a->h, "rpcf_set_state", &a->set_state) && sym(a->h, "rpcf_get_state", &a->get_state) &&
            sym(a->h, "rpcf_get_nonlinear", &a->get_nonlinear) && sym(a->h, "rpcf_set_nonlinear", &a->set_nonlinear) &&
            sym(a->h, "rpcf_step", &a->step) && sym(a->h, "rpcf_bootstrap", &a->bootstrap) &&
-           sym(a->h, "rpcf_seed_state", &a->seed_state);
+           sym(a->h, "rpcf_seed_state", &a->seed_state) && sym(a->h, "rpcf_get_field", &a->get_field) &&
+           sym(a->h, "rpcf_phys_to_spec", &a->phys_to_spec) && sym(a->h, "rpcf_spec_to_phys", &a->spec_to_phys);
 }
 
 struct Config {
     rpcf_params prm{};
     int start_type = 0, start_step = 0, end_step = 0, save_internal = 1, save_recovery_internal = 1;
     std::string output_prefix, recovery_prefix;
+    bool seeded = false;  // line 8 present: seeded field; absent: the reference's analytic field
     uint64_t seed = 0;
     double amplitude = 1e-2, kexp = 0.0;
     int kmax = 4, degmax = 8;
@@ -124,6 +136,7 @@ bool read_para(const std::string& file, Config* c) {
             std::cerr << "rpcf_run: line 8 must be: seed amplitude kmax degmax kexp" << std::endl;
             return false;
         }
+        c->seeded = true;
     }
     return true;
 }
@@ -132,6 +145,88 @@ struct Sizes {
     int nx, ny, nz, pz, hnx;
     size_t P;
 };
+
+// The reference's analytic initial field (init_flow_kernel and the _get_init_* profiles,
+// source/RPCFKernels.cu:221-356, amplitude EPSILON_INIT = 0.005 at source/RPCFKernels.cu:132):
+// wall-normal velocity w and wall-normal vorticity omega_z at the padded x/span points and
+// the nz collocation planes z_p = cos(pi p/(nz-1)) (initFlow passes nz as the plane count,
+// source/RPCFKernels.cu:372-376), zero on the two wall planes, then the unpadded forward
+// transform (source/RPCFKernels.cu:401-406).  The (0,0) pencils -- the mean of w and
+// omega_z, zero for this field -- become u0 and w0 as in initFlow (409-418).
+struct InitProfile {
+    double lx, ly;
+    static constexpr double kEps = 0.005;
+    double w(double x, double y, double z) const {
+        const double pi = 3.14159265358979323846;
+        const double cx1 = 2 * pi * x / lx, cx2 = 4 * pi * x / lx, cy1 = 2 * pi * y / ly, cy2 = 4 * pi * y / ly;
+        return kEps * (-2.0 / 3.0) * (1.0 + cos(1.5 * pi * z)) *
+               (sin(cx1) * sin(cy1) + sin(cx2) * sin(cy1) + sin(cx1) * sin(cy2));
+    }
+    double omega_z(double x, double y, double z) const {
+        const double pi = 3.14159265358979323846;
+        const double cx1 = 2 * pi * x / lx, cx2 = 4 * pi * x / lx, cy1 = 2 * pi * y / ly, cy2 = 4 * pi * y / ly;
+        return kEps * 2.0 * pi * sin(1.5 * pi * z) *
+               (lx / ly * (cos(cx1) * cos(cy1) + 0.5 * cos(cx2) * cos(cy1) + 2.0 * cos(cx1) * cos(cy2)) +
+                ly / lx * (0.5 * cos(cx1) * cos(cy1) + cos(cx2) * cos(cy1) + 0.25 * cos(cx1) * cos(cy2)));
+    }
+};
+
+bool reference_initial_state(const Api& a, rpcf_ctx* ctx, const rpcf_params& p, int nz, size_t P,
+                             std::vector<double>* v, std::vector<double>* e) {
+    const double pi = 3.14159265358979323846;
+    // lx = 2 pi aphi, ly = 2 pi beta with aphi = n_pi_x/2, beta = n_pi_y/2 (source/data.cpp:5-6)
+    const InitProfile prof{pi * p.n_pi_x, pi * p.n_pi_y};
+    const size_t plane = (size_t)p.mx * p.my;
+    std::vector<double> fw(plane * nz, 0.0), fo(plane * nz, 0.0);
+    for (int k = 1; k < nz - 1; k++) {
+        const double z = cos(pi * k / (nz - 1));
+        for (int j = 0; j < p.my; j++) {
+            const double y = (double)j / p.my * prof.ly;
+            for (int i = 0; i < p.mx; i++) {
+                const double x = (double)i / p.mx * prof.lx;
+                fw[plane * k + (size_t)p.mx * j + i] = prof.w(x, y, z);
+                fo[plane * k + (size_t)p.mx * j + i] = prof.omega_z(x, y, z);
+            }
+        }
+    }
+    v->assign(2 * P * nz, 0.0);
+    e->assign(2 * P * nz, 0.0);
+    return a.phys_to_spec(ctx, fw.data(), v->data(), nz, 0) == 0 && a.phys_to_spec(ctx, fo.data(), e->data(), nz, 0) == 0;
+}
+
+// output_velocity / write_velocity (source/output.cpp:18-103): the fields u, v, w (file
+// <prefix>_velocity.<step>.dat) and omega_x, omega_y, omega_z (<prefix>_vortricity.<step>.dat,
+// the reference's spelling) on the padded physical grid, raw doubles, field after field, each
+// as pz planes of my rows of mx values.  Also prints the maximum magnitude of every field, as
+// statistics() does (source/statistics.cpp:13-64).
+bool write_velocity_files(const Api& a, rpcf_ctx* ctx, const Config& c, const Sizes& z, int step) {
+    const rpcf_params& p = c.prm;
+    const size_t n = 2 * z.P * z.nz, np = (size_t)z.pz * p.my * p.mx;
+    std::vector<double> spec(n), phys(np);
+    static const char* const kNames[6] = {"u", "v", "w", "ox", "oy", "oz"};
+    std::ofstream out;
+    for (int f = 0; f < 6; f++) {
+        if (a.get_field(ctx, f, spec.data(), z.nz) || a.spec_to_phys(ctx, spec.data(), phys.data(), z.nz, 1))
+            return false;
+        double mx = 0.0;
+        for (double x : phys) mx = std::max(mx, std::fabs(x));
+        std::cout << "max " << kNames[f] << ":" << mx << std::endl;
+        if (c.output_prefix.empty() || c.output_prefix == "-") continue;
+        if (f % 3 == 0) {
+            out.close();
+            const std::string file = c.output_prefix + (f == 0 ? "_velocity." : "_vortricity.") + std::to_string(step) + ".dat";
+            std::cout << "writing results to " << file << std::endl;
+            out.open(file, std::ios::binary);
+            if (!out) {
+                std::cerr << "Cannot open output file:" << file << std::endl;
+                return false;
+            }
+        }
+        out.write((const char*)phys.data(), sizeof(double) * np);
+        if (!out) return false;
+    }
+    return true;
+}
 
 const char kMagic[8] = {'R', 'P', 'C', 'F', 'R', 'C', 'V', '1'};
 
@@ -249,11 +344,17 @@ int main(int argc, char** argv) {
         std::cout << "init flow" << std::endl;
         const size_t n = 2 * z.P * z.nz;
         std::vector<double> v(n), e(n);
-        if (api.seed_state(&cfg.prm, cfg.seed, cfg.amplitude, cfg.kmax, cfg.degmax, cfg.kexp, v.data(), e.data(), z.nz))
-            return fail("seeded initial field");
+        if (cfg.seeded) {
+            if (api.seed_state(&cfg.prm, cfg.seed, cfg.amplitude, cfg.kmax, cfg.degmax, cfg.kexp, v.data(), e.data(),
+                               z.nz))
+                return fail("seeded initial field");
+        } else if (!reference_initial_state(api, ctx, cfg.prm, z.nz, z.P, &v, &e)) {
+            return fail("analytic initial field");
+        }
         if (api.set_state(ctx, v.data(), e.data(), z.nz)) return fail("set_state");
         std::cout << "output initial flow field" << std::endl;
         if (!write_snapshot(api, ctx, cfg, z, 0, u00)) return fail("snapshot");
+        if (!write_velocity_files(api, ctx, cfg, z, 0)) return fail("velocity output");
         // extra previous step + re-initialisation + real first step (source/main_loop.cpp:39-52)
         std::cout << "first step" << std::endl;
         if (api.bootstrap(ctx)) return fail("bootstrap");
@@ -281,6 +382,7 @@ int main(int argc, char** argv) {
         if (cfg.save_internal > 0 && i % cfg.save_internal == 0) {
             std::cout << "[OUTPUT] WRITING RESULTS" << std::endl;
             if (!write_snapshot(api, ctx, cfg, z, i, u00)) return fail("snapshot");
+            if (!write_velocity_files(api, ctx, cfg, z, i)) return fail("velocity output");
             std::cout << "velocity(0,0,0):" << u00[0] << " " << u00[1] << std::endl;
         }
         if (cfg.save_recovery_internal > 0 && i % cfg.save_recovery_internal == 0 && i > 0) {

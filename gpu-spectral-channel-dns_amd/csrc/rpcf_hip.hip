@@ -104,6 +104,7 @@ struct rpcf_ctx {
     std::vector<int> g_ids;
     // per-kernel timing with hipEvents on the context stream
     int timing = 0;
+    unsigned timing_mask = ~0u;      // kernels (bit kid) bracketed by events while timing
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
     double t_ms[KID_COUNT] = {0};
@@ -209,7 +210,7 @@ int sync_ctx(rpcf_ctx* c) {
 template <typename F>
 int launch(rpcf_ctx* c, int kid, F&& fn, bool capturing = false) {
     hipEvent_t a = nullptr, b = nullptr;
-    const bool timed = c->timing && !capturing;
+    const bool timed = c->timing && ((c->timing_mask >> kid) & 1u) && !capturing;
     if (timed) {
         a = get_event(c);
         (void)hipEventRecord(a, c->stream);
@@ -1272,11 +1273,14 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
 }
 
 // ---- per-kernel timing (hipEvents on the context's stream) ----------------------------
-int rpcf_timing(rpcf_ctx* c, int enable) {
+int rpcf_timing(rpcf_ctx* c, int enable) { return rpcf_timing_kernels(c, enable ? ~0u : 0u); }
+
+int rpcf_timing_kernels(rpcf_ctx* c, unsigned mask) {
     if (!c) return RPCF_EINVAL;
     int rc = sync_ctx(c);
     if (rc) return rc;
-    c->timing = enable ? 1 : 0;
+    c->timing = mask ? 1 : 0;
+    c->timing_mask = mask;
     for (int k = 0; k < KID_COUNT; k++) { c->t_ms[k] = 0.0; c->t_cnt[k] = 0; }
     return RPCF_OK;
 }

@@ -160,6 +160,9 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
  * (0 <= kid < rpcf_num_kernels(), named by rpcf_kernel_name).  The oracle implements
  * the same symbols and reports no kernels. */
 int rpcf_timing(rpcf_ctx* ctx, int enable);
+/* The same with events around the kernels whose bit (1u << kid) is set in `mask` only, so
+ * that a timed run pays for the events of the kernels it measures; 0 disables. */
+int rpcf_timing_kernels(rpcf_ctx* ctx, unsigned mask);
 int rpcf_timing_get(rpcf_ctx* ctx, int kid, double* total_ms, long* count);
 const char* rpcf_kernel_name(int kid);
 int rpcf_num_kernels(void);

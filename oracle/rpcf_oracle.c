@@ -1255,6 +1255,7 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
 
 /* profiling symbols of include/rpcf.h: the oracle has no device kernels */
 int rpcf_timing(rpcf_ctx* c, int enable) { (void)enable; return c ? RPCF_OK : RPCF_EINVAL; }
+int rpcf_timing_kernels(rpcf_ctx* c, unsigned mask) { (void)mask; return c ? RPCF_OK : RPCF_EINVAL; }
 int rpcf_timing_get(rpcf_ctx* c, int kid, double* total_ms, long* count) {
     (void)c; (void)kid; (void)total_ms; (void)count;
     return RPCF_EINVAL;

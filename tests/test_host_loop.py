@@ -85,3 +85,92 @@ def test_host_loop_errors(tmp_path):
     open(para, "w").write("24 24 64\n2.0 1.0\n")
     r = run(rpcf.ORACLE_LIB, para)
     assert r.returncode == 2
+
+
+# ---- the reference's analytic initial field and its output files (start_type 0, no line 8)
+
+def analytic_w_oz(x, y, z, lx, ly):
+    """w and omega_z of init_flow_kernel (source/RPCFKernels.cu:258-301), EPSILON_INIT 0.005"""
+    eps, pi = 0.005, np.pi
+    cx1, cx2, cy1, cy2 = 2 * pi * x / lx, 4 * pi * x / lx, 2 * pi * y / ly, 4 * pi * y / ly
+    w = eps * (-2.0 / 3.0) * (1 + np.cos(1.5 * pi * z)) * (
+        np.sin(cx1) * np.sin(cy1) + np.sin(cx2) * np.sin(cy1) + np.sin(cx1) * np.sin(cy2))
+    oz = eps * 2 * pi * np.sin(1.5 * pi * z) * (
+        lx / ly * (np.cos(cx1) * np.cos(cy1) + 0.5 * np.cos(cx2) * np.cos(cy1) + 2 * np.cos(cx1) * np.cos(cy2))
+        + ly / lx * (0.5 * np.cos(cx1) * np.cos(cy1) + np.cos(cx2) * np.cos(cy1) + 0.25 * np.cos(cx1) * np.cos(cy2)))
+    return w, oz
+
+
+def expected_padded(f_nodes, nz, pz):
+    """the field given at the nz Gauss-Lobatto planes (walls zero) as a Chebyshev
+    interpolant (numpy's chebfit, exact at nz nodes) evaluated on the pz padded planes"""
+    zn = np.cos(np.pi * np.arange(nz) / (nz - 1))
+    zp = np.cos(np.pi * np.arange(pz) / (pz - 1))
+    cols = f_nodes.reshape(nz, -1)
+    coef = np.polynomial.chebyshev.chebfit(zn, cols, nz - 1)
+    return np.polynomial.chebyshev.chebval(zp, coef).T.reshape((pz,) + f_nodes.shape[1:])
+
+
+def run_reference_ic(lib, tmp_path, start=0, end=0):
+    para = str(tmp_path / "parameter.txt")
+    # the reference's six lines only: base flow (1+y)/2, FP64, analytic initial field.  The
+    # step-0 files are written after initialisation and again after loop step 0 (as the
+    # reference does); start = 1, end = 0 runs no loop step and keeps the initial ones.
+    lines = ["24 24 64", "2.0 1.0", "500 0.0 0.01", f"0 {start} {end} 1 100", str(tmp_path / "a"),
+             str(tmp_path / "a_rec")]
+    open(para, "w").write("\n".join(lines) + "\n")
+    r = run(lib, para)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def read_fields(path, pz, my, mx):
+    a = np.fromfile(path, np.float64)
+    assert a.size == 3 * pz * my * mx
+    return a.reshape(3, pz, my, mx)
+
+
+def test_reference_initial_field_files_on_oracle(tmp_path):
+    r = run_reference_ic(rpcf.ORACLE_LIB, tmp_path, start=1, end=0)
+    assert "STEP:" not in r.stdout
+    mx, my, nz, pz = 24, 24, 17, 33
+    lx, ly = 2.0 * np.pi, 1.0 * np.pi
+    x = np.arange(mx) / mx * lx
+    y = np.arange(my) / my * ly
+    zn = np.cos(np.pi * np.arange(nz) / (nz - 1))
+    Z, Y, X = np.meshgrid(zn, y, x, indexing="ij")
+    w, oz = analytic_w_oz(X, Y, Z, lx, ly)
+    w[0], w[-1], oz[0], oz[-1] = 0.0, 0.0, 0.0, 0.0  # the kernel zeroes the wall planes
+    vel = read_fields(str(tmp_path / "a_velocity.0.dat"), pz, my, mx)
+    vor = read_fields(str(tmp_path / "a_vortricity.0.dat"), pz, my, mx)
+    for got, ref in ((vel[2], expected_padded(w, nz, pz)), (vor[2], expected_padded(oz, nz, pz))):
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    # u and v come from velocity recovery: continuity du/dx + dv/dy + dw/dz = 0 on the
+    # padded grid, checked spectrally in x and span and with numpy's chebder in z
+    u, v = vel[0], vel[1]
+    kx = 2 * np.pi / lx * np.fft.fftfreq(mx, 1.0 / mx)
+    ky = 2 * np.pi / ly * np.fft.fftfreq(my, 1.0 / my)
+    du = np.fft.ifft(1j * kx * np.fft.fft(u, axis=2), axis=2).real
+    dv = np.fft.ifft(1j * ky[:, None] * np.fft.fft(v, axis=1), axis=1).real
+    zp = np.cos(np.pi * np.arange(pz) / (pz - 1))
+    cw = np.polynomial.chebyshev.chebfit(zp, vel[2].reshape(pz, -1), pz - 1)
+    dw = np.polynomial.chebyshev.chebval(zp, np.polynomial.chebyshev.chebder(cw)).T.reshape(pz, my, mx)
+    assert np.abs(u).max() > 1e-4 and np.abs(du + dv + dw).max() < 1e-9 * np.abs(dw).max()
+    # the statistics lines: max |field| over the padded grid
+    for name, f in (("u", u), ("v", v), ("w", vel[2]), ("oz", vor[2])):
+        line = [l for l in r.stdout.splitlines() if l.startswith(f"max {name}:")][0]
+        assert abs(float(line.split(":")[1]) - np.abs(f).max()) <= 1e-5 * np.abs(f).max()
+
+
+@pytest.mark.gpu
+def test_reference_initial_field_files_gpu_matches_oracle(tmp_path):
+    (tmp_path / "o").mkdir()
+    (tmp_path / "g").mkdir()
+    run_reference_ic(rpcf.ORACLE_LIB, tmp_path / "o", end=5)
+    run_reference_ic(rpcf.HIP_LIB, tmp_path / "g", end=5)
+    for step in (0, 5):
+        for kind in ("velocity", "vortricity"):
+            a = read_fields(str(tmp_path / "g" / f"a_{kind}.{step}.dat"), 33, 24, 24)
+            b = read_fields(str(tmp_path / "o" / f"a_{kind}.{step}.dat"), 33, 24, 24)
+            for f in range(3):
+                assert np.abs(a[f] - b[f]).max() <= 1e-11 * np.abs(b[f]).max(), (step, kind, f)
