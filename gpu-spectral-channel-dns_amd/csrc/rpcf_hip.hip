@@ -63,6 +63,7 @@ struct rpcf_ctx {
     hipStream_t stream = nullptr;
     int lds_limit = 65536;
     int ncu = 256;                   // compute units of the device
+    long matrix_cache_hits = 0;      // slots whose inverses were copied from another slot
     // tiles: sequences per block and LDS row stride (elements) of each transform kernel
     int TB_s2p = 1, TB_nl = 1, TB_nl2 = 0, TB_vel = 1, TK_inv = 1, TK_fwd = 1, RP = 1;
     int ld_zn = 0, ld_my = 0, ld_mx = 0, zld = 0;
@@ -799,8 +800,27 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     RCCHK(zero_sync(c, c->SV + slot * c->sS, sizeof(double2) * c->sS));
     RCCHK(zero_sync(c, c->SE + slot * c->sS, sizeof(double2) * c->sS));
 
-    // collocation matrices and inverses, in chunks of matrix slots (initAuxMatrix)
+    // Matrix cache: the inverses depend on (Lx, Lz, Re, dt, U) only -- not on Ro, the flow
+    // control or the state -- so a slot of this context already configured with the same
+    // operator parameters lends its inverses (a device copy instead of the LU build).
     const size_t per = nz * nz;
+    double2* minv = c->MINV + slot * c->sM;
+    for (int s2 = 0; s2 < c->capacity; s2++) {
+        const SlotHost& o = c->slots[s2];
+        if (s2 == slot || !o.configured) continue;
+        const rpcf_params& a = o.prm;
+        if (a.n_pi_x == prm->n_pi_x && a.n_pi_y == prm->n_pi_y && a.Re == prm->Re && a.dt == prm->dt &&
+            a.u_profile[0] == prm->u_profile[0] && a.u_profile[1] == prm->u_profile[1] &&
+            a.u_profile[2] == prm->u_profile[2]) {
+            HIPCHK(c, hipMemcpyAsync(minv, c->MINV + s2 * c->sM, sizeof(double2) * 2 * (size_t)d.nslot * per,
+                                     hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            c->matrix_cache_hits++;
+            sh.configured = 1;
+            return RPCF_OK;
+        }
+    }
+    // collocation matrices and inverses, in chunks of matrix slots (initAuxMatrix)
     const int chunk =
         (int)std::min<size_t>((size_t)d.nslot, std::max<size_t>(64, ((size_t)1 << 29) / (2 * per * sizeof(double2))));
     double2* work = nullptr;
@@ -808,7 +828,6 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     if ((rc = dalloc(c, &work, 2 * (size_t)chunk * per))) return rc;
     if ((rc = dalloc(c, &status, 1))) { (void)hipFree(work); return rc; }
     if ((rc = zero_sync(c, status, sizeof(int)))) { (void)hipFree(work); (void)hipFree(status); return rc; }
-    double2* minv = c->MINV + slot * c->sM;
     for (int s0 = 0; s0 < d.nslot && !rc; s0 += chunk) {
         dim3 g(std::min(chunk, d.nslot - s0), 2);
         k_build_inverse<<<g, 256, 0, c->stream>>>(minv, work, s0, d, lp, c->T0, c->T2, c->T4, c->U + slot * c->sU,
@@ -1324,6 +1343,12 @@ int rpcf_member_init(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     if (rc) return rc;
     c->slots[slot].configured = 0;
     return init_slot(c, slot, prm);
+}
+
+int rpcf_group_cache_hits(const rpcf_ctx* c, long* hits) {
+    if (!c || !hits) return RPCF_EINVAL;
+    *hits = c->matrix_cache_hits;
+    return RPCF_OK;
 }
 
 int rpcf_member_release(rpcf_ctx* c, int slot) {

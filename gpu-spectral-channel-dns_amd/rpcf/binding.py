@@ -112,6 +112,7 @@ def load(path):
         "rpcf_group_bootstrap": (ctypes.c_int, [_vp, ip, ctypes.c_int]),
         "rpcf_group_step": (ctypes.c_int, [_vp, ip, ctypes.c_int, ctypes.c_int]),
         "rpcf_group_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "rpcf_group_cache_hits": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long)]),
     }
     for name, (res, args) in list(sig.items()) + list(group_sig.items()):
         fn = getattr(dll, name, None)
@@ -339,3 +340,9 @@ class Group:
 
     def graphs(self, enable=True):
         self._check(self.dll.rpcf_group_graphs(self.h, int(bool(enable))), "group_graphs")
+
+    def cache_hits(self):
+        """member initialisations whose inverses were copied from another slot"""
+        n = ctypes.c_long()
+        self._check(self.dll.rpcf_group_cache_hits(self.h, ctypes.byref(n)), "group_cache_hits")
+        return n.value

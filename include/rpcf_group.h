@@ -34,9 +34,14 @@ int rpcf_slot_bytes(const rpcf_params* shape, size_t* bytes);
 
 /* Put a member into a slot: its base profile, operators and the inverses of its
  * Crank-Nicolson matrices are built (initAuxMatrix, source/solver.cpp:260-320); state and
- * history are cleared.  rpcf_member_release empties the slot. */
+ * history are cleared.  rpcf_member_release empties the slot.  The inverses depend on
+ * (n_pi_x, n_pi_y, Re, dt, u_profile) only, not on Ro, flow_control or the state: when a
+ * configured slot of the group has the same values (bit for bit), its inverses are copied
+ * instead of built (the matrix cache; a rotation-number sweep builds them once). */
 int rpcf_member_init(rpcf_ctx* group, int slot, const rpcf_params* prm);
 int rpcf_member_release(rpcf_ctx* group, int slot);
+/* Number of member initialisations served by the matrix cache since the group was created. */
+int rpcf_group_cache_hits(const rpcf_ctx* group, long* hits);
 /* rpcf_set_state / rpcf_get_state of one slot. */
 int rpcf_member_set_state(rpcf_ctx* group, int slot, const double* v_hat, const double* eta_hat,
                           long pencil_stride);

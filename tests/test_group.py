@@ -110,3 +110,28 @@ def test_group_errors():
             g.step([0, 0], 1)  # listed twice
         with pytest.raises(rpcf.RpcfError):
             g.step([1], 1)  # empty slot
+
+
+def test_rotation_sweep_shares_inverses():
+    """members that differ only in Ro, flow control and seed take their inverses from the
+    matrix cache (copied from the first slot) and still match the same simulations alone"""
+    base = dict(MEMBERS[0])
+    sweep = [dict(base, Ro=ro, flow_control=fc) for ro, fc in ((0.0, 0), (0.15, 0), (0.3, 1))]
+    prms = [rpcf.make_params(**SHAPE, **m) for m in sweep]
+    states = [rpcf.seed_state(rpcf.load(rpcf.HIP_LIB), p, 200 + i, *IC) for i, p in enumerate(prms)]
+    with rpcf.Group(rpcf.HIP_LIB, prms[0], len(prms)) as g:
+        for i, p in enumerate(prms):
+            g.init(i, p)
+            g.set_state(i, *states[i])
+        assert g.cache_hits() == len(prms) - 1
+        slots = list(range(len(prms)))
+        g.bootstrap(slots)
+        g.step(slots, 4)
+        got = [g.get_state(i) for i in slots]
+    for i, p in enumerate(prms):
+        with rpcf.Simulation(rpcf.HIP_LIB, p) as sim:
+            sim.set_state(*states[i])
+            sim.bootstrap()
+            sim.step(4)
+            ref = sim.get_state()
+        assert np.array_equal(got[i][0], ref[0]) and np.array_equal(got[i][1], ref[1]), i
