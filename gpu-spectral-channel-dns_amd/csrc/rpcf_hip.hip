@@ -10,10 +10,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/rpcf.h"
@@ -545,6 +549,91 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
         cap);
 }
 
+// ---- tile autotuner ------------------------------------------------------------------
+// The tile of each transform kernel only regroups independent pencils, rows or columns, so
+// every candidate gives bit-identical results and the choice is free to follow the clock.
+// At context creation the candidates (the largest tile that fits, and half of it) are timed
+// with HIP events on slot 0 (zero state, placeholder physics) by coordinate descent over
+// the kernels; a candidate replaces the current one when it is 3 % faster.  The choice is
+// kept per (mx, my, mz, precision) for the life of the process.  RPCF_AUTOTUNE=0 keeps the
+// rule-based tiles; RPCF_AUTOTUNE=half takes the halved candidates untimed (for tests).
+struct TileSet {
+    int TB_s2p, TK_inv, TK_fwd, RP, TB_nl2, TB_vel;
+};
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, TileSet> g_tuned;
+
+int* tile_ref(rpcf_ctx* c, int i) {
+    int* refs[6] = {&c->TB_s2p, &c->TK_inv, &c->TK_fwd, &c->RP, &c->TB_nl2, &c->TB_vel};
+    return refs[i];
+}
+constexpr int kTileKernel[6] = {KID_S2P, KID_SPAN_INV, KID_SPAN_FWD, KID_LAMB, KID_P2S_NL, KID_VELOCITY};
+
+int autotune_tiles(rpcf_ctx* c) {
+    const char* env = getenv("RPCF_AUTOTUNE");
+    const std::string mode = env ? env : "1";
+    if (mode == "0") return RPCF_OK;
+    const Dims d = c->d;
+    bool compiled = false;
+#define RPCF_CASE(L) compiled |= 2 * d.N == L;
+    RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+    // k_velocity_tile and k_p2s_nl2 run for compiled Chebyshev lengths only
+    const bool tunable[6] = {true, true, true, true, c->TB_nl2 > 0, compiled};
+    if (mode == "half") {
+        for (int i = 0; i < 6; i++)
+            if (tunable[i] && *tile_ref(c, i) >= 2) *tile_ref(c, i) /= 2;
+        return RPCF_OK;
+    }
+    const std::array<int, 4> key = {d.mx, d.my, d.N, c->mixed};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            const TileSet& t = it->second;
+            c->TB_s2p = t.TB_s2p, c->TK_inv = t.TK_inv, c->TK_fwd = t.TK_fwd, c->RP = t.RP;
+            c->TB_nl2 = t.TB_nl2, c->TB_vel = t.TB_vel;
+            return RPCF_OK;
+        }
+    }
+    // slot 0 with zero fields and placeholder physics (init_slot overwrites all of it)
+    MemberParams mp{};
+    mp.aphi = mp.beta = 1.0;
+    mp.Re = 1000.0;
+    mp.dt = 1e-3;
+    RCCHK(copy_sync(c, c->MP, &mp, sizeof(MemberParams), hipMemcpyHostToDevice));
+    RCCHK(zero_sync(c, c->F, sizeof(double2) * c->sF));
+    RCCHK(zero_sync(c, c->SV, sizeof(double2) * c->sS));
+    RCCHK(zero_sync(c, c->SE, sizeof(double2) * c->sS));
+    RCCHK(zero_sync(c, c->NL, sizeof(double2) * c->sNL));
+    auto time_kernel = [&](int kid, double* ms) -> int {
+        c->timing = 1;
+        c->timing_mask = 1u << kid;
+        for (int k = 0; k < KID_COUNT; k++) c->t_ms[k] = 0.0, c->t_cnt[k] = 0;
+        int rc = RPCF_OK;
+        for (int r = 0; r < 3 && !rc; r++) rc = enq_nonlinear(c, c->ids_zero, 1, true);
+        if (!rc) rc = sync_ctx(c);
+        c->timing = 0;
+        c->timing_mask = ~0u;
+        *ms = c->t_cnt[kid] ? c->t_ms[kid] / c->t_cnt[kid] : 0.0;
+        return rc;
+    };
+    for (int i = 0; i < 6; i++) {
+        int* t = tile_ref(c, i);
+        if (!tunable[i] || *t < 2) continue;
+        const int cur = *t;
+        double t_cur = 0.0, t_alt = 0.0;
+        RCCHK(time_kernel(kTileKernel[i], &t_cur));
+        *t = cur / 2;
+        RCCHK(time_kernel(kTileKernel[i], &t_alt));
+        if (!(t_alt < 0.97 * t_cur)) *t = cur;
+    }
+    RCCHK(zero_sync(c, c->PH, sizeof(int)));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = TileSet{c->TB_s2p, c->TK_inv, c->TK_fwd, c->RP, c->TB_nl2, c->TB_vel};
+    return RPCF_OK;
+}
+
 void free_ctx(rpcf_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -729,9 +818,10 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_x_r2c, lim);
     allow_lds(k_lin_tv<kTvPencils>, lim);
     (void)hipGetLastError();
+    RCCHK(autotune_tiles(c));
     if (getenv("RPCF_VERBOSE"))
-        fprintf(stderr, "rpcf: lds_limit=%d TB_s2p=%d TB_nl=%d TK=%d RP=%d nslot=%d capacity=%ld\n", lim, c->TB_s2p,
-                c->TB_nl, c->TK_inv, c->RP, d.nslot, C);
+        fprintf(stderr, "rpcf: lds_limit=%d TB_s2p=%d TB_nl2=%d TB_vel=%d TK=%d/%d RP=%d nslot=%d capacity=%ld\n", lim,
+                c->TB_s2p, c->TB_nl2, c->TB_vel, c->TK_inv, c->TK_fwd, c->RP, d.nslot, C);
     return RPCF_OK;
 }
 
