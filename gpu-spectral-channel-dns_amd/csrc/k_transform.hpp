@@ -178,6 +178,150 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
     else body(IntC<0>{});
 }
 
+// LDS elements (double2) of k_cheb_s2p_h for a tile of tb pencils
+__host__ __device__ constexpr size_t s2p_h_lds(int fn, int tb) {
+    return 2 * (size_t)tb * (fn / 2 + 2) + (size_t)tb * (fn / 4 + 2) + fn + fn / 4;
+}
+
+// -------------------------------------------------------------------------------------
+// k_cheb_s2p (PAD = 1) for compiled 2N >= 64 with half-length transforms.  With a_k, k <= N,
+// the padded values f_q = sum_k a_k cos(pi k q/(2N)), q = 0..2N, split as
+//   q odd, q = 2m+1:  O_m = DCT-III_N(a_0..a_{N-1})_m                 (a_N drops out)
+//   q = 4p:           E_2p = DCT-I_{N/2+1}(b)_p,  b_0 = a_0 + a_N, b_k = a_k + a_{N-k}, b_{N/2} = a_{N/2}
+//   q = 4p+2:         E_2p+1 = DCT-III_{N/2}(c)_p, c_k = a_k - a_{N-k}
+// DCT-I_{M+1}(b) is the length-2M FFT of the even extension (s_0 = b_0, s_M = b_M,
+// s_k = s_{2M-k} = b_k/2).  DCT-III_M(X)_n = sum_{k<M} X_k cos(pi k (2n+1)/(2M)) for complex X
+// is one length-M inverse FFT (Makhoul's reordering): Z_0 = 2 X_0,
+// Z_k = e^{i pi k/(2M)} (X_k - i X_{M-k}); z = sum_k Z_k e^{2 pi i k n/M}; then
+// x_2n = z_n / 2, x_2n+1 = z_{M-1-n} / 2.  FFT work per pencil: N + N + N/2 points in place of
+// 2N + 2N.  The twiddle tables of lengths N and N/2 are taken from tw4 (exp(-2 pi i k/(4N)))
+// into LDS.  LDS: A [2 TB][N+2] (rows [0, nb): Z of the odd points; rows [nb, 2nb): the even
+// extension of b; rows [TB, TB+nb) stage the coefficients first), B [TB][N/2+2], tables.
+// -------------------------------------------------------------------------------------
+template <int FN, typename RO = double>
+__global__ __launch_bounds__(256) void k_cheb_s2p_h(const double2* __restrict__ F, long fstride, long sF,
+                                                    cplx<RO>* __restrict__ Y, long ystride, long sY, Dims d,
+                                                    const double2* __restrict__ tw4, int TB,
+                                                    const int* __restrict__ ids) {
+    static_assert(FN >= 64 && FN % 8 == 0, "2N >= 64, N divisible by 4");
+    RPCF_LDS(double2);
+    constexpr int N = FN / 2, H = N / 2, nz = N + 1, pz = FN + 1, LA = N + 2, LB = H + 2;
+    constexpr int TBX = s2p_tile_max(FN);
+    const int f = blockIdx.z % 6, kx = blockIdx.y, ky0 = blockIdx.x * TB;
+    const long slot = ids[blockIdx.z / 6];
+    F += slot * sF;
+    Y += slot * sY;
+    const int nb_rt = min(TB, d.ny - ky0), ny = d.ny, tid = threadIdx.x, nthr = blockDim.x;
+    // a full tile of width s2p_tile_max(FN) makes the tile width a constant
+    auto body = [&](auto nbc) {
+    constexpr int NB = decltype(nbc)::value;
+    const int nb = NB ? NB : nb_rt;
+    double2* A = lds;
+    double2* B = A + 2 * TB * LA;
+    double2* twN = B + TB * LB;
+    double2* twH = twN + N;
+    double2* pre = twH + H;  // tw4[k], k < N (conjugated where used)
+    for (int i = tid; i < N; i += nthr) {
+        twN[i] = tw4[4 * i];
+        pre[i] = tw4[i];
+    }
+    for (int i = tid; i < H; i += nthr) twH[i] = tw4[8 * i];
+    const int pstride = d.hnx * nz;
+    const double2* in = F + (f * fstride + (ky0 * d.hnx + kx) * nz);
+    double2* stg = A + TB * LA;
+    staged<8>(
+        nb * nz, tid, nthr,
+        [&](int i) {
+            const unsigned b = (unsigned)i / nz, k = i - b * nz;
+            return in[b * pstride + k];
+        },
+        [&](int i, double2 v) {
+            const unsigned b = (unsigned)i / nz, k = i - b * nz;
+            stg[b * LA + k] = v;
+        });
+    __syncthreads();
+    // Z of the odd points (M = N) and Z' of the points 4p+2 (M = N/2); the even extension
+    // of b into registers (its rows overlap the staging area)
+    for (int i = tid; i < nb * N; i += nthr) {
+        const unsigned b = (unsigned)i / N, k = i - b * N;
+        const double2* a = stg + b * LA;
+        double2 z;
+        if (k == 0) {
+            z = make_double2(2.0 * a[0].x, 2.0 * a[0].y);
+        } else {
+            const double2 p = a[k], m = a[N - k], w = pre[k];
+            const double2 x = make_double2(p.x + m.y, p.y - m.x);  // X_k - i X_{N-k}
+            z = make_double2(w.x * x.x + w.y * x.y, w.x * x.y - w.y * x.x);  // conj(w) x
+        }
+        A[b * LA + k] = z;
+    }
+    for (int i = tid; i < nb * H; i += nthr) {
+        const unsigned b = (unsigned)i / H, k = i - b * H;
+        const double2* a = stg + b * LA;
+        double2 z;
+        if (k == 0) {
+            z = make_double2(2.0 * (a[0].x - a[N].x), 2.0 * (a[0].y - a[N].y));
+        } else {
+            const double2 ck = make_double2(a[k].x - a[N - k].x, a[k].y - a[N - k].y);
+            const double2 cm = make_double2(a[H - k].x - a[H + k].x, a[H - k].y - a[H + k].y);
+            const double2 x = make_double2(ck.x + cm.y, ck.y - cm.x);
+            const double2 w = pre[2 * k];
+            z = make_double2(w.x * x.x + w.y * x.y, w.x * x.y - w.y * x.x);
+        }
+        B[b * LB + k] = z;
+    }
+    constexpr int KS = (TBX * N + 255) / 256;  // even-extension elements per thread
+    double2 sv[KS];
+#pragma unroll
+    for (int u = 0; u < KS; u++) {
+        const int i = tid + u * nthr;
+        if (i < nb * N) {
+            const unsigned b = (unsigned)i / N, k = i - b * N;
+            const double2* a = stg + b * LA;
+            if (k == 0) sv[u] = make_double2(a[0].x + a[N].x, a[0].y + a[N].y);
+            else if (k == H) sv[u] = a[H];
+            else sv[u] = make_double2(0.5 * (a[k].x + a[N - k].x), 0.5 * (a[k].y + a[N - k].y));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KS; u++) {
+        const int i = tid + u * nthr;
+        if (i < nb * N) {
+            const unsigned b = (unsigned)i / N, k = i - b * N;
+            A[(nb + b) * LA + k] = sv[u];
+        }
+    }
+    __syncthreads();
+    FftPlan pn{}, ph{};
+    pn.tw = twN;
+    ph.tw = twH;
+    fft_any<N, +1, 2 * TBX>(A, LA, 2 * nb, pn, tid, nthr);
+    fft_any<H, +1, TBX>(B, LB, nb, ph, tid, nthr);
+    cplx<RO>* out = Y + (f * ystride + kx * pz * ny + ky0);
+    staged<8>(
+        pz * nb, tid, nthr,
+        [&](int i) {
+            const unsigned q = (unsigned)i / nb, b = i - q * nb;
+            if (q & 1) {
+                const unsigned m = q >> 1;
+                const double2 z = A[b * LA + ((m & 1) ? N - 1 - (m >> 1) : (m >> 1))];
+                return make_double2(0.5 * z.x, 0.5 * z.y);
+            }
+            const unsigned p = q >> 2;
+            if ((q & 3) == 0) return A[(nb + b) * LA + p];
+            const double2 z = B[b * LB + ((p & 1) ? H - 1 - (p >> 1) : (p >> 1))];
+            return make_double2(0.5 * z.x, 0.5 * z.y);
+        },
+        [&](int i, double2 v) {
+            const unsigned q = (unsigned)i / nb, b = i - q * nb;
+            out[q * ny + b] = to_c<cplx<RO>>(v);
+        });
+    };
+    if (nb_rt == TBX) body(IntC<TBX>{});
+    else body(IntC<0>{});
+}
+
 // -------------------------------------------------------------------------------------
 // Span inverse FFT with the 3/2 zero padding: the span C2C half of the reference's 2D
 // Z2D (source/transform.cu:176) after transpose_backward + setZeros

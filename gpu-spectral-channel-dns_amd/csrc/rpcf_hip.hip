@@ -393,7 +393,15 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
             c, KID_S2P,
             [&] {
                 switch (2 * d.N) {
-#define RPCF_CASE(L) case L: k_cheb_s2p<1, L, R><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, Y, yS, c->sY, d, c->pl_zn, tw4, TB, ld, ids); break;
+#define RPCF_CASE(L)                                                                                             \
+    case L:                                                                                                      \
+        if constexpr (L >= 64)                                                                                   \
+            k_cheb_s2p_h<(L >= 64 ? L : 64), R><<<g, kFftThreads, sizeof(double2) * s2p_h_lds(L, TB), c->stream>>>( \
+                c->F, fS, c->sF, Y, yS, c->sY, d, tw4, TB, ids);                                                 \
+        else                                                                                                     \
+            k_cheb_s2p<1, L, R><<<g, kFftThreads, lds, c->stream>>>(c->F, fS, c->sF, Y, yS, c->sY, d, c->pl_zn, tw4, \
+                                                                   TB, ld, ids);                                 \
+        break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
@@ -755,7 +763,10 @@ int init_ctx(rpcf_ctx* c) {
 #define RPCF_CASE(L) if (2 * d.N == L) tmax = s2p_tile_max(L);
         RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-        c->TB_s2p = pick_tile(tmax, 2 * d.N, c->cap_zn, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_zn; });
+        // room for k_cheb_s2p and for k_cheb_s2p_h (the half-length form, 2N >= 64)
+        c->TB_s2p = pick_tile(tmax, 2 * d.N, c->cap_zn, lim3, [&](int t) {
+            return sizeof(double2) * std::max((size_t)t * c->ld_zn, s2p_h_lds(2 * d.N, t));
+        });
     }
     c->TB_nl = pick_tile(kChebTile, 2 * d.N, c->cap_zn, lim3, [&](int t) {
         return sizeof(double2) * ((size_t)t * c->ld_zn + 256);
@@ -809,6 +820,8 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_lamb_phys<L, float>, lim);                \
     allow_lds(k_p2s_nl<L, float>, lim);                   \
     allow_lds(k_p2s_nl2<L>, lim);                         \
+    allow_lds(k_cheb_s2p_h<(L >= 64 ? L : 64)>, lim);      \
+    allow_lds(k_cheb_s2p_h<(L >= 64 ? L : 64), float>, lim); \
     allow_lds(k_p2s_nl2<L, float>, lim);                  \
     allow_lds(k_velocity_tile<L>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
