@@ -266,8 +266,10 @@ struct LinState {
     double2* sv;          // v-hat (u0 in pencil 0), updated in place
     double2* se;          // eta-hat (w0 in pencil 0)
     const double2* minv;  // [2][nslot][nz*nz] column-major inverses
+    const double2* minvh; // [2][nslot][np][nz] their parity blocks (k_parity_blocks)
     double qw_target_re, qw_target_im;  // flow-rate target (flow_control)
     int flow_control;
+    int parity;           // 1: solve with the parity blocks
 };
 
 // -------------------------------------------------------------------------------------
@@ -487,6 +489,112 @@ __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const dou
     }
 }
 
+// The same with a rows x cols matrix (column-major, M[c*rows + r]) whose rows r < split
+// take the vectors xa and the others xb.
+__device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const double2* xa0, const double2* xa1,
+                                         const double2* xb0, const double2* xb1, int rows, int cols, int split,
+                                         double2* part, int tid, int nthr, double2* y0, double2* y1) {
+    constexpr int B = 8;
+    const int G = nthr / rows;
+    const int g = tid / rows, r = tid - g * rows;
+    const double2* x0 = r < split ? xa0 : xb0;
+    const double2* x1 = r < split ? xa1 : xb1;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g < G) {
+        int j = g;
+        for (; j + G * (B - 1) < cols; j += G * B) {
+            double2 m[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) m[u] = M[(long)(j + G * u) * rows + r];
+#pragma unroll
+            for (int u = 0; u < B; u++) gemv_acc(m[u], x0[j + G * u], x1[j + G * u], a);
+        }
+        for (; j < cols; j += G) gemv_acc(M[(long)j * rows + r], x0[j], x1[j], a);
+        part[tid] = make_double2(a[0], a[1]);
+        part[kSolveThreads + tid] = make_double2(a[2], a[3]);
+    }
+    __syncthreads();
+    if (tid < rows) {
+        double2 s0 = part[tid], s1 = part[kSolveThreads + tid];
+        for (int q = 1; q < G; q++) {
+            s0 = zadd(s0, part[q * rows + tid]);
+            s1 = zadd(s1, part[kSolveThreads + q * rows + tid]);
+        }
+        *y0 = s0;
+        *y1 = s1;
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// Parity blocks of the inverses of one slot (block (matrix slot, equation)).  With an even
+// base profile the collocation system A of either equation commutes with the wall reflection:
+// A R = Q A, (R c)_j = (-1)^j c_j on the Chebyshev coefficients and Q the signed pairing of
+// the rows (pairs[p] = (i, i', s): the rows of z_p and z_{N-p}, the two wall rows of each
+// condition, s = -1 for the slope rows).  Hence M = A^-1 has M[j][i'] = (-1)^j s M[j][i]:
+// the even coefficients of x = M b depend on b_i + s b_i' only and the odd ones on
+// b_i - s b_i', and
+//   x_j = sum_p H[j][p] (b_i + (-1)^j s b_i')   (a centre row i = i' enters once, and only
+//                                                 for even j)
+// with H[j][p] = (M[j][i] + (-1)^j s M[j][i'])/2 -- half the bytes of M.  Computed from the
+// full inverse, so the blocks agree with it to its own (round-off) asymmetry.  Layout of a
+// block: np columns (pairs) of nz rows, the even j = 0, 2, .. first, then the odd j.
+// -------------------------------------------------------------------------------------
+__global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __restrict__ minvh,
+                                const int4* __restrict__ pairs, int np, Dims d) {
+    const int slot = blockIdx.x, eq = blockIdx.y, n = d.nz, ne = d.N / 2 + 1;
+    const double2* M = minv + ((long)eq * d.nslot + slot) * n * n;  // M[j][i] at i*n + j
+    double2* Hm = minvh + ((long)eq * d.nslot + slot) * (long)n * np;
+    const int4* pr = pairs + eq * np;
+    for (long e = threadIdx.x; e < (long)n * np; e += blockDim.x) {
+        const int p = (int)(e / n), r = (int)(e - (long)p * n);
+        const int4 q = pr[p];
+        const bool even = r < ne;
+        const int j = even ? 2 * r : 2 * (r - ne) + 1;
+        double2 v;
+        if (q.x == q.y) {
+            v = even ? M[(long)q.x * n + j] : make_double2(0.0, 0.0);
+        } else {
+            const double2 a = M[(long)q.x * n + j], b = M[(long)q.y * n + j];
+            const double sg = even ? (double)q.z : -(double)q.z;
+            v = make_double2(0.5 * (a.x + sg * b.x), 0.5 * (a.y + sg * b.y));
+        }
+        Hm[e] = v;
+    }
+}
+
+// x = M b for the two right-hand sides rhs[0..1][nz] of a slot: with the full inverse M, or,
+// for a member with an even base profile, with its parity blocks H (k_parity_blocks): the
+// combined sides b_i + s b_i' and b_i - s b_i' per row pair, then one nz x np GEMV whose even
+// rows take the first and odd rows the second.  Thread r < nz returns x at coefficient
+// *jrow.  Every thread of the block calls it (it synchronises).
+__device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const int4* pairs, int np, int parity,
+                                           double2 (*rhs)[256], double2 (*bc)[2][128], double2* part, int n, int N,
+                                           int tid, int nthr, double2* y0, double2* y1, int* jrow) {
+    if (!parity) {
+        gemv_grouped(M, rhs[0], rhs[1], n, part, tid, nthr, y0, y1);
+        *jrow = tid;
+        return;
+    }
+    if (tid < np) {
+        const int4 q = pairs[tid];
+        for (int v = 0; v < 2; v++) {
+            const double2 a = rhs[v][q.x], b = rhs[v][q.y];
+            if (q.x == q.y) {
+                bc[v][0][tid] = a;
+                bc[v][1][tid] = make_double2(0.0, 0.0);
+            } else {
+                const double s = (double)q.z;
+                bc[v][0][tid] = make_double2(a.x + s * b.x, a.y + s * b.y);
+                bc[v][1][tid] = make_double2(a.x - s * b.x, a.y - s * b.y);
+            }
+        }
+    }
+    __syncthreads();
+    const int ne = N / 2 + 1;
+    gemv_sel(Hm, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1);
+    *jrow = tid < ne ? 2 * tid : 2 * (tid - ne) + 1;
+}
+
 // -------------------------------------------------------------------------------------
 // Crank-Nicolson/AB2 update of the one or two pencils of a matrix slot from the operator
 // products W (k_lin_tv): v right-hand side (_get_linear_v + AB2, source/linear.cpp:86-125,
@@ -502,11 +610,14 @@ __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const dou
 __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restrict__ SV, double2* __restrict__ SE, long sS,
                                                    const double2* __restrict__ NL, long sNL,
                                                    const double2* __restrict__ MINV, long sM,
+                                                   const double2* __restrict__ MINVH, long sMH,
+                                                   const int4* __restrict__ pairs, int np,
                                                    const double2* __restrict__ W, long sWl, Dims d, Members mb,
                                                    const double* __restrict__ T0t, const double* __restrict__ U,
                                                    const double* __restrict__ dU, const double* __restrict__ ddU, long sU,
                                                    const double* __restrict__ qw) {
     __shared__ double2 rhs[2][256], vnew[2][256], part[2][kSolveThreads];
+    __shared__ double2 bc[2][2][128];  // parity form: [vector][b_i + s b_i', b_i - s b_i'][pair]
     __shared__ double2 s_f;
     LinState st;
     LinParams lp;
@@ -523,6 +634,8 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
         st.sv = SV + ms * sS;
         st.se = SE + ms * sS;
         st.minv = MINV + ms * sM;
+        st.minvh = MINVH + ms * sMH;
+        st.parity = mp.parity;
         st.qw_target_re = mp.q_re;
         st.qw_target_im = mp.q_im;
         st.flow_control = mp.flow_control;
@@ -616,10 +729,13 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
         rhs[1][r] = a1;
     }
     __syncthreads();
+    const long nh = (long)n * np;
     {
         double2 y0, y1;
-        gemv_grouped(Mv, rhs[0], rhs[1], n, &part[0][0], r, blockDim.x, &y0, &y1);
-        if (r < n) vnew[0][r] = y0, vnew[1][r] = y1;
+        int j;
+        solve_slot(Mv, st.minvh + (long)slot * nh, pairs, np, st.parity, rhs, bc, &part[0][0], n, d.N, r,
+                   blockDim.x, &y0, &y1, &j);
+        if (r < n) vnew[0][j] = y0, vnew[1][j] = y1;
     }
     __syncthreads();
     {
@@ -650,12 +766,14 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
     }
     __syncthreads();
     double2 e0, e1;
-    gemv_grouped(Mo, rhs[0], rhs[1], n, &part[0][0], r, blockDim.x, &e0, &e1);
+    int je;
+    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, pairs + np, np, st.parity, rhs, bc, &part[0][0], n, d.N,
+               r, blockDim.x, &e0, &e1, &je);
     if (r < n) {
-        st.se[po0 + r] = e0;
+        st.se[po0 + je] = e0;
         st.sv[po0 + r] = vnew[0][r];
         if (two) {
-            st.se[po1 + r] = e1;
+            st.se[po1 + je] = e1;
             st.sv[po1 + r] = vnew[1][r];
         }
     }

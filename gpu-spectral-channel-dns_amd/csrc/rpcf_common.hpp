@@ -31,6 +31,7 @@ struct MemberParams {
     double aphi, beta, Re, dt, Ro;
     double q_re, q_im;
     int flow_control;
+    int parity;  // 1: even base profile, the solves read the parity blocks of the inverses
 };
 
 // Member addressing of one grouped launch: launch member j runs slot ids[j]; a per-slot

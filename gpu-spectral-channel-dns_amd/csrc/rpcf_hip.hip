@@ -87,6 +87,9 @@ struct rpcf_ctx {
     double2* Z2 = nullptr;    // [3][pz][my][zld], zld = hnx rounded up to 8 (128-byte rows)
     double2* WL = nullptr;    // [5][P][nz] operator products of the linear right-hand sides
     double2* MINV = nullptr;  // [2][nslot][nz*nz] inverses, column-major
+    double2* MINVH = nullptr; // [2][nslot][np][nz] parity blocks of the inverses (even base profiles)
+    int4* PAIRS = nullptr;    // [2][np] row pairs (i, i', sign, 0) of the v and eta systems
+    int npair = 0;            // np = N/2 + 1
     double* TF = nullptr;     // [5][RT][S][64] MFMA A-operand tiles of T0, T2, T4q, T0, T2q
     double* T4Q = nullptr;    // [nz*nz] T4*dt*0.5/Re, transposed (fallback operator products)
     double* T2Q = nullptr;    // [nz*nz] T2*0.5/Re*dt, transposed
@@ -94,7 +97,7 @@ struct rpcf_ctx {
     double* MEANU = nullptr;                             // [pz] base profile at the padded points
     MemberParams* MP = nullptr;                          // [capacity]
     int* PH = nullptr;                                   // [capacity] history phase
-    long sF = 0, sS = 0, sNL = 0, sY = 0, sW = 0, sZ = 0, sWl = 0, sM = 0, sTf = 0, sQ = 0, sU = 0, sPz = 0;
+    long sF = 0, sS = 0, sNL = 0, sY = 0, sW = 0, sZ = 0, sWl = 0, sM = 0, sMH = 0, sTf = 0, sQ = 0, sU = 0, sPz = 0;
     // shape tables shared by all slots
     double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *T0t = nullptr, *T2t = nullptr, *qw = nullptr, *tw = nullptr;
     std::vector<double> T0h, T2h, T4h, qw_host;
@@ -158,14 +161,16 @@ int check_sizes(const rpcf_params* prm, Dims* out) {
 
 // per-slot element counts of every per-slot array (= the slot strides)
 struct SlotSizes {
-    long F, S, NL, Y, W, Z, Wl, M, Tf, Q, U, Pz;
+    long F, S, NL, Y, W, Z, Wl, M, MH, Tf, Q, U, Pz;
     size_t bytes() const {
-        return sizeof(double2) * (size_t)(F + 2 * S + NL + Y + W + Z + Wl + M) +
+        return sizeof(double2) * (size_t)(F + 2 * S + NL + Y + W + Z + Wl + M + MH) +
                sizeof(double) * (size_t)(Tf + 2 * Q + 3 * U + Pz) + sizeof(MemberParams) + 2 * sizeof(int);
     }
 };
 
-SlotSizes slot_sizes(const Dims& d) {
+// parity blocks: kept for the mixed form only (the FP64 form solves with the full inverses,
+// whose rounding the reference shares; the blocks agree with them to ~1e-11 after a step)
+SlotSizes slot_sizes(const Dims& d, int precision) {
     SlotSizes s;
     const long P = d.P, nz = d.nz, zld = (d.hnx + 7) & ~7;
     const long RT = (nz + 15) / 16, S = (nz + 3) / 4;
@@ -177,6 +182,7 @@ SlotSizes slot_sizes(const Dims& d) {
     s.Z = 3L * d.pz * d.my * zld;
     s.Wl = 5 * P * nz;
     s.M = 2L * d.nslot * nz * nz;
+    s.MH = precision ? 2L * d.nslot * nz * (d.N / 2 + 1) : 0;  // nz rows x (N/2 + 1) pair columns
     s.Tf = 5 * RT * S * 64;
     s.Q = nz * nz;
     s.U = nz;
@@ -551,6 +557,7 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
             // about four column groups of nz threads (all of kSolveThreads at nz = 129)
             const int thr = std::min(kSolveThreads, 64 * ((4 * d.nz + 63) / 64));
             k_lin_solve<<<dim3(d.nslot, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
+                                                                 c->MINVH, c->sMH, c->PAIRS, c->npair,
                                                                  c->WL, c->sWl, d, members(c, ids), c->T0t, c->U, c->DU,
                                                                  c->DDU, c->sU, c->qw);
         },
@@ -648,7 +655,7 @@ void free_ctx(rpcf_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     void* ptrs[] = {c->F,   c->SV, c->SE, c->NL,  c->Y,     c->W,  c->Z2, c->WL, c->MINV, c->TF,  c->T4Q,
                     c->T2Q, c->U,  c->DU, c->DDU, c->MEANU, c->MP, c->PH, c->T0, c->T2,   c->T4,  c->T0t,
-                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf};
+                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->PAIRS};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -680,16 +687,17 @@ int init_ctx(rpcf_ctx* c) {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
         c->ncu = ncu;
 
-    const SlotSizes ss = slot_sizes(d);
+    const SlotSizes ss = slot_sizes(d, c->shape.precision);
     c->sF = ss.F; c->sS = ss.S; c->sNL = ss.NL; c->sY = ss.Y; c->sW = ss.W; c->sZ = ss.Z;
-    c->sWl = ss.Wl; c->sM = ss.M; c->sTf = ss.Tf; c->sQ = ss.Q; c->sU = ss.U; c->sPz = ss.Pz;
+    c->sWl = ss.Wl; c->sM = ss.M; c->sMH = ss.MH; c->sTf = ss.Tf; c->sQ = ss.Q; c->sU = ss.U; c->sPz = ss.Pz;
     c->zld = (d.hnx + 7) & ~7;
     c->tv_rt = (d.nz + 15) / 16;
     c->tv_s = (d.nz + 3) / 4;
     if ((rc = dalloc(c, &c->F, C * ss.F)) || (rc = dalloc(c, &c->SV, C * ss.S)) || (rc = dalloc(c, &c->SE, C * ss.S)) ||
         (rc = dalloc(c, &c->NL, C * ss.NL)) || (rc = dalloc(c, &c->Y, C * ss.Y)) || (rc = dalloc(c, &c->W, C * ss.W)) ||
         (rc = dalloc(c, &c->Z2, C * ss.Z)) || (rc = dalloc(c, &c->WL, C * ss.Wl)) ||
-        (rc = dalloc(c, &c->MINV, C * ss.M)) || (rc = dalloc(c, &c->TF, C * ss.Tf)) ||
+        (rc = dalloc(c, &c->MINV, C * ss.M)) || (rc = dalloc(c, &c->MINVH, C * ss.MH)) ||
+        (rc = dalloc(c, &c->TF, C * ss.Tf)) ||
         (rc = dalloc(c, &c->T4Q, C * ss.Q)) || (rc = dalloc(c, &c->T2Q, C * ss.Q)) ||
         (rc = dalloc(c, &c->U, C * ss.U)) || (rc = dalloc(c, &c->DU, C * ss.U)) || (rc = dalloc(c, &c->DDU, C * ss.U)) ||
         (rc = dalloc(c, &c->MEANU, C * ss.Pz)) || (rc = dalloc(c, &c->MP, (size_t)C)) ||
@@ -703,6 +711,26 @@ int init_ctx(rpcf_ctx* c) {
     if ((rc = dalloc(c, &c->tw, 2 * ((size_t)d.mx + d.my + 4 * d.N + 2 * d.N)))) return rc;
     RCCHK(zero_sync(c, c->ids_zero, sizeof(int)));
     RCCHK(zero_sync(c, c->PH, sizeof(int) * C));
+    {
+        // Row pairs of the collocation systems under the wall reflection y -> -y (the rows of
+        // z_p and z_{N-p}; the two wall rows of each condition).  v (4th order): rows 0, 1
+        // (value at the walls) and 2, 3 (slope at the walls, which changes sign), rows
+        // 4..N at the points p = i - 2; eta (2nd order): rows 0, 1, rows 2..N at p = i - 1.
+        // A row paired with itself (the centre point) has i == i'.
+        const int N = d.N, np = N / 2 + 1;
+        std::vector<int4> pr(2 * (size_t)np);
+        int k = 0;
+        pr[k++] = make_int4(0, 1, 1, 0);
+        pr[k++] = make_int4(2, 3, -1, 0);
+        for (int i = 4; i <= N + 4 - i; i++) pr[k++] = make_int4(i, N + 4 - i, 1, 0);
+        if (k != np) return fail(c, RPCF_EINVAL, "v row pairs: %d, expected %d", k, np);
+        pr[k++] = make_int4(0, 1, 1, 0);
+        for (int i = 2; i <= N + 2 - i; i++) pr[k++] = make_int4(i, N + 2 - i, 1, 0);
+        if (k != 2 * np) return fail(c, RPCF_EINVAL, "eta row pairs: %d, expected %d", k - np, np);
+        c->npair = np;
+        if ((rc = dalloc(c, &c->PAIRS, pr.size()))) return rc;
+        RCCHK(copy_sync(c, c->PAIRS, pr.data(), sizeof(int4) * pr.size(), hipMemcpyHostToDevice));
+    }
     c->slots.assign(C, SlotHost());
 
     // shape operators (initAuxMatrix, source/solver.cpp:265-269)
@@ -841,6 +869,15 @@ int init_ctx(rpcf_ctx* c) {
 // Physics of one slot: base profile, per-slot operator tables and the inverses of its
 // Crank-Nicolson collocation matrices (initAuxMatrix, source/solver.cpp:260-320); the slot's
 // state, fields and history are cleared.
+// parity blocks of a slot's inverses (k_parity_blocks), on the context stream
+int build_parity_blocks(rpcf_ctx* c, int slot) {
+    const Dims d = c->d;
+    k_parity_blocks<<<dim3(d.nslot, 2), 256, 0, c->stream>>>(c->MINV + slot * c->sM, c->MINVH + slot * c->sMH, c->PAIRS,
+                                                             c->npair, d);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RPCF_OK : fail(c, RPCF_EDEVICE, "k_parity_blocks: %s", hipGetErrorString(e));
+}
+
 int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     const Dims d = c->d;
     const size_t nz = d.nz;
@@ -861,6 +898,9 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     sh.mp.Ro = prm->Ro;
     sh.mp.q_re = sh.mp.q_im = 0.0;
     sh.mp.flow_control = prm->flow_control;
+    // an even base profile U(y) (no linear part) keeps the collocation systems symmetric under
+    // y -> -y, so the solves of the mixed form use the parity blocks of the inverses
+    sh.mp.parity = c->shape.precision != 0 && prm->u_profile[1] == 0.0 ? 1 : 0;
 
     std::vector<double> U(nz), dU(nz), ddU(nz), mU(d.pz);
     rpcf_host::base_profile(d.nz, prm->u_profile, U.data(), dU.data(), ddU.data());
@@ -917,6 +957,7 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
             a.u_profile[2] == prm->u_profile[2]) {
             HIPCHK(c, hipMemcpyAsync(minv, c->MINV + s2 * c->sM, sizeof(double2) * 2 * (size_t)d.nslot * per,
                                      hipMemcpyDeviceToDevice, c->stream));
+            if (sh.mp.parity) RCCHK(build_parity_blocks(c, slot));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             c->matrix_cache_hits++;
             sh.configured = 1;
@@ -944,6 +985,8 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     (void)hipFree(work);
     (void)hipFree(status);
     if (!rc && hstatus) rc = fail(c, RPCF_EINVAL, "%d singular collocation matrices", hstatus);
+    if (!rc && sh.mp.parity) rc = build_parity_blocks(c, slot);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, RPCF_EDEVICE, "parity blocks failed");
     if (!rc) sh.configured = 1;
     return rc;
 }
@@ -1432,7 +1475,7 @@ int rpcf_group_capacity(const rpcf_ctx* c) { return c ? c->capacity : 0; }
 int rpcf_slot_bytes(const rpcf_params* shape, size_t* bytes) {
     Dims d;
     if (!bytes || check_sizes(shape, &d)) return RPCF_EINVAL;
-    *bytes = slot_sizes(d).bytes();
+    *bytes = slot_sizes(d, shape->precision).bytes();
     return RPCF_OK;
 }
 
