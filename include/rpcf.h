@@ -50,7 +50,10 @@ typedef struct rpcf_ctx rpcf_ctx;
  *                 1: mixed (HIP library, compiled FFT lengths only) -- the transform
  *                 intermediates, the x and span FFTs and the Lamb vector in FP32; the
  *                 Chebyshev transforms, the nonlinear combination, the operators and the
- *                 solves in FP64.  The oracle computes FP64 whatever this field says. */
+ *                 solves in FP64, the solves of an even base profile (u_profile[1] == 0)
+ *                 with the reflection-parity blocks of the inverses (half their bytes;
+ *                 they agree with the full inverses to round-off of the inverse).  The
+ *                 oracle computes FP64 whatever this field says. */
 typedef struct rpcf_params {
     int mx, my, mz;
     double n_pi_x, n_pi_y;

@@ -193,3 +193,36 @@ def test_orr_sommerfeld_growth_rate():
 
     sigma = growth_rate_run(rpcf.ORACLE_LIB)
     assert abs(sigma / 0.00373967 - 1) < 1e-4, sigma
+
+
+def test_inverse_parity_structure():
+    """the structure the mixed form's parity-block solves rely on (k_parity_blocks): with an
+    even base profile the inverse of either collocation system satisfies
+    M[j][i'] = (-1)^j s M[j][i] for the row pairs (i, i', s) of the wall reflection -- v: rows
+    0/1 (+), 2/3 (slopes, -), i/N+4-i; eta: rows 0/1, i/N+2-i -- to round-off; an odd profile
+    (Couette) breaks it"""
+    def pairs(eq, N):
+        if eq == 0:
+            pr = [(0, 1, 1), (2, 3, -1)] + [(i, N + 4 - i, 1) for i in range(4, N + 5 - 4) if i <= N + 4 - i]
+        else:
+            pr = [(0, 1, 1)] + [(i, N + 2 - i, 1) for i in range(2, N + 1) if i <= N + 2 - i]
+        assert len(pr) == N // 2 + 1
+        return pr
+
+    def asym(prm, eq):
+        with rpcf.Simulation(rpcf.ORACLE_LIB, prm) as sim:
+            s = sim.s
+            N = s["nz"] - 1
+            worst = 0.0
+            sign = (-1.0) ** np.arange(N + 1)
+            for p in (1, s["hnx"] + 3, 5 * s["hnx"] + 2):
+                M = sim.get_inverse(eq, p)
+                dev = max(np.abs(M[:, i2] - sign * sg * M[:, i]).max() for i, i2, sg in pairs(eq, N))
+                worst = max(worst, dev / np.abs(M).max())
+            return worst
+
+    channel = params("C0", u_profile=(1.0, 0.0, -1.0), flow_control=1)
+    couette = params("C0", u_profile=(0.0, 1.0, 0.0))
+    for eq in (0, 1):
+        assert asym(channel, eq) < 1e-12
+    assert asym(couette, 0) > 1e-6
