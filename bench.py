@@ -112,12 +112,13 @@ def copy_probe(nbytes, reps):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
-def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64):
-    """the same workload in the mixed-precision form (FP32 transform intermediates, x/span
-    FFTs and Lamb vector; FP64 Chebyshev transforms, operators and solves): time per step
-    and the relative field error against the FP64 run after the same steps"""
+def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64, precision=1):
+    """the same workload in another form -- precision 1, the mixed form (FP32 transform
+    intermediates, x/span FFTs and Lamb vector; FP64 Chebyshev transforms, operators and
+    parity-block solves), or 2 (FP64 with the parity-block solves): time per step and the
+    relative field error against the FP64 run after the same steps"""
     prm = rpcf.make_params(**C1)
-    prm.precision = 1
+    prm.precision = precision
     with rpcf.Simulation(rpcf.HIP_LIB, prm) as sim:
         sim.set_state(v0, e0)
         sim.bootstrap()
@@ -130,9 +131,10 @@ def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64):
         v, e = sim.get_state()
     err = {"v": float(np.abs(v - final64[0]).max() / np.abs(final64[0]).max()),
            "eta": float(np.abs(e - final64[1]).max() / np.abs(final64[1]).max())}
-    log(f"mixed: {ms:.4f} ms/step ({ms64 / ms:.2f}x FP64), error after {1 + warmup + steps} steps {err}")
+    log(f"precision {precision}: {ms:.4f} ms/step ({ms64 / ms:.2f}x FP64), error after {1 + warmup + steps} steps {err}")
     return {"ms_per_step": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3), "steps_compared": 1 + warmup + steps,
-            "max_rel_error": err, "dtype": "f32 transforms / f64 solves"}
+            "max_rel_error": err,
+            "dtype": "f32 transforms / f64 solves" if precision == 1 else "f64, parity-block solves"}
 
 
 def ensemble_leg(n, nsteps):
@@ -274,6 +276,8 @@ def main():
     # the mixed run takes the same steps as the FP64 run (bootstrap, warmup, table pass, timed)
     mixed = None if args.no_mixed else mixed_leg(prm, v0, e0, args.warmup + args.table_steps, args.steps, ms_step,
                                                  final64)
+    fp64_parity = None if args.no_mixed else mixed_leg(prm, v0, e0, args.warmup + args.table_steps, args.steps,
+                                                       ms_step, final64, precision=2)
 
     ens = ensemble_leg(args.ensemble, args.ensemble_steps) if args.ensemble else None
     srv = serving_leg(args.serve, args.serve_horizon, tuple(args.serve_steps)) if args.serve > 0 else None
@@ -310,6 +314,7 @@ def main():
         "parity_1step_vs_oracle": parity,
         "kernels": kernels,
         "mixed": mixed,
+        "fp64_parity_solves": fp64_parity,
         "ensemble": ens,
         "serving": srv,
     }

@@ -143,7 +143,7 @@ int check_sizes(const rpcf_params* prm, Dims* out) {
     const int mx = prm->mx, my = prm->my, mz = prm->mz;
     if (mx < 6 || my < 6 || mz < 16 || mx % 2 || my % 2 || mz % 4) return RPCF_EINVAL;
     if (!(prm->Re > 0) || !(prm->dt > 0) || !(prm->n_pi_x > 0) || !(prm->n_pi_y > 0)) return RPCF_EINVAL;
-    if (prm->precision != 0 && prm->precision != 1) return RPCF_EINVAL;
+    if (prm->precision < 0 || prm->precision > 2) return RPCF_EINVAL;
     Dims d;
     d.mx = mx; d.my = my;
     d.nx = mx / 3 * 2; d.ny = my / 3 * 2;

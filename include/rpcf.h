@@ -52,8 +52,11 @@ typedef struct rpcf_ctx rpcf_ctx;
  *                 Chebyshev transforms, the nonlinear combination, the operators and the
  *                 solves in FP64, the solves of an even base profile (u_profile[1] == 0)
  *                 with the reflection-parity blocks of the inverses (half their bytes;
- *                 they agree with the full inverses to round-off of the inverse).  The
- *                 oracle computes FP64 whatever this field says. */
+ *                 they agree with the full inverses to round-off of the inverse).
+ *                 2: FP64 with those parity-block solves (even base profiles; an odd
+ *                 profile runs as 0): the state agrees with precision 0 and the reference
+ *                 to about 1e-11 after one step instead of 1e-15.  The oracle computes FP64
+ *                 with full inverses whatever this field says. */
 typedef struct rpcf_params {
     int mx, my, mz;
     double n_pi_x, n_pi_y;

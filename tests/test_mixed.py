@@ -3,6 +3,8 @@ FFTs and the Lamb vector in FP32, the Chebyshev transforms, the nonlinear combin
 the operators and the solves in FP64.  Bar: after 100 steps (200 for the rotating Couette
 case) every field stays within 1e-6 relative (max |a-b| / max |b|) of the FP64 run from
 the same initial field, and differs from it (the FP32 path ran)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -30,9 +32,17 @@ def run(prm, ic, steps):
     return state, fields
 
 
+def fresh(prm):
+    """a copy of a parameter set (the cases are shared between tests)"""
+    q = type(prm)()
+    ctypes.pointer(q)[0] = prm
+    return q
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_mixed_within_1e6_of_fp64(name):
     prm, ic, steps = CASES[name]
+    prm = fresh(prm)
     (v64, e64), f64 = run(prm, ic, steps)
     prm.precision = 1
     (v32, e32), f32 = run(prm, ic, steps)
@@ -49,3 +59,26 @@ def test_group_precision_must_match():
         q.precision = 1
         with pytest.raises(rpcf.RpcfError):
             g.init(0, q)
+
+
+def test_fp64_parity_solves():
+    """precision 2: FP64 with the reflection-parity blocks of the inverses for an even base
+    profile -- within 1e-10 of the full-inverse FP64 state after one step and 1e-8 after 20
+    (the blocks drop the inverse's own rounding asymmetry); a Couette (odd) profile runs the
+    full inverses, bit for bit as precision 0"""
+    prm, ic, _ = CASES["C1"]
+    prm = fresh(prm)
+    a, _ = run(prm, ic, 1)
+    a20, _ = run(prm, ic, 20)
+    prm.precision = 2
+    b, _ = run(prm, ic, 1)
+    b20, _ = run(prm, ic, 20)
+    e1, e20 = max(rel(b[0], a[0]), rel(b[1], a[1])), max(rel(b20[0], a20[0]), rel(b20[1], a20[1]))
+    print("precision 2 vs 0: 1 step %.2e, 20 steps %.2e" % (e1, e20))
+    assert 0 < e1 < 1e-10 and e20 < 1e-8
+    prm0, ic0, _ = CASES["C0"]  # plane Couette
+    prm0 = fresh(prm0)
+    c, _ = run(prm0, ic0, 3)
+    prm0.precision = 2
+    d, _ = run(prm0, ic0, 3)
+    assert np.array_equal(c[0], d[0]) and np.array_equal(c[1], d[1])
