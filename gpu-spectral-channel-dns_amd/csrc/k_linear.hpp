@@ -360,23 +360,28 @@ __global__ __launch_bounds__(384) void k_lin_tv(const double2* __restrict__ sv, 
 // The same operator products on the matrix cores: W[m] = T_m X with X the nz x 2P real
 // matrix of (Re, Im) columns, by v_mfma_f64_16x16x4_f64 (A: lane l holds A[l%16][l/16],
 // B: lane l holds B[l/16][l%16], D: item r of lane l is D[l/16 + 4r][l%16]).
+// The Chebyshev tables have the reflection parity T_m[N-i][j] = (-1)^j T_m[i][j] (m = 0, 2,
+// 4 at the symmetric points), so with E_i = sum_{j even} T_m[i][j] x_j and
+// O_i = sum_{j odd} T_m[i][j] x_j for the rows i <= N/2,
+//   W[i] = E_i + O_i,   W[N-i] = E_i - O_i
+// -- half the matrix-core work of the full product.  Tf[m][par][rt][s][64]: the 16x4 tile
+// (rows 16rt.., columns 4s.. of the even (par 0: j = 2k) or odd (par 1: j = 2k+1) columns)
+// of T_m in A-operand lane order, zero outside, so a chunk is one contiguous copy.
 // Block: operator m = blockIdx.y % 5 of member blockIdx.y / 5, 32 pencils, 4 waves of 8
-// pencils (16 columns).  A wave
-// keeps its B fragments (S = ceil(nz/4) k-steps) in registers; the A fragments of
-// kTvChunk row tiles are staged in LDS once per block and chunk.  Tf[m][rt][s][64]: the
-// 16x4 tile (rows 16rt.., columns 4s..) of T_m in A-operand lane order, zero outside
-// nz x nz, so a chunk is one contiguous copy; X beyond nz (and beyond P) enters as zeros.
+// pencils (16 columns); a wave keeps its B fragments (even and odd coefficients, S k-steps
+// each) in registers; the A fragments of kTvChunk row tiles of both parities are staged in
+// LDS once per block and chunk.
 // -------------------------------------------------------------------------------------
 typedef double rpcf_v4d __attribute__((ext_vector_type(4)));
 constexpr int kTvChunk = 3;
 constexpr int kTvBlockPencils = 32;
 
-template <int S>
+template <int S>  // k-steps per parity: S >= ceil((N/2 + 1)/4)
 __global__ __launch_bounds__(256) void k_lin_tv_mfma(const double2* __restrict__ sv, const double2* __restrict__ se,
                                                      long sS, double2* __restrict__ W, long sWl, Dims d,
                                                      const double* __restrict__ Tf, long sTf, int RT,
                                                      const int* __restrict__ ids) {
-    extern __shared__ __align__(16) double tvA[];  // [kTvChunk][S][64]
+    extern __shared__ __align__(16) double tvA[];  // [2][kTvChunk][S][64]
     const int m = blockIdx.y % 5, tid = threadIdx.x, lane = tid & 63;
     {
         const long slot = ids[blockIdx.y / 5];
@@ -385,53 +390,66 @@ __global__ __launch_bounds__(256) void k_lin_tv_mfma(const double2* __restrict__
         W += slot * sWl;
         Tf += slot * sTf;
     }
-    const int n = d.nz, col = lane & 15, kq = lane >> 4;
+    const int n = d.nz, N = d.N, col = lane & 15, kq = lane >> 4;
     const int p = blockIdx.x * kTvBlockPencils + (tid >> 6) * 8 + (col >> 1);
     const bool okp = p < d.P;
     const double* xp = (const double*)((m < 3 ? sv : se) + (long)(okp ? p : 0) * n) + (col & 1);
-    double b[S];
+    double be[S], bo[S];
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        // unconditional load from a clamped address, masked by a multiply (a select lets
-        // the compiler sink the load into a branch that waits on it)
-        const int k = 4 * s + kq;
-        b[s] = xp[2 * min(k, n - 1)] * ((okp && k < n) ? 1.0 : 0.0);
+        // unconditional loads from clamped addresses, masked by a multiply (a select lets
+        // the compiler sink the loads into a branch that waits on them)
+        const int je = 2 * (4 * s + kq), jo = je + 1;
+        be[s] = xp[2 * min(je, n - 1)] * ((okp && je < n) ? 1.0 : 0.0);
+        bo[s] = xp[2 * min(jo, n - 1)] * ((okp && jo < n) ? 1.0 : 0.0);
     }
-    const double2* A = (const double2*)(Tf + (long)m * RT * S * 64);
+    const double2* A = (const double2*)(Tf + (long)m * 2 * RT * S * 64);
     double* Wm = (double*)(W + (long)m * d.P * n) + (long)p * 2 * n + (col & 1);
+    const int rows = N / 2 + 1;
     for (int rc = 0; rc < RT; rc += kTvChunk) {
-        const int cnt = min(kTvChunk, RT - rc) * S * 32;  // double2 elements of the chunk
-        const double2* src = A + (long)rc * S * 32;
+        const int ct = min(kTvChunk, RT - rc);
+        const int cnt = ct * S * 32;  // double2 elements of one parity's chunk
         __syncthreads();
         constexpr int KC = (kTvChunk * S * 32 + 255) / 256;
-        double vx[KC], vy[KC];
 #pragma unroll
-        for (int u = 0; u < KC; u++) {
-            const int i = min(tid + u * 256, cnt - 1);
-            const double2 t = src[i];
-            vx[u] = t.x;
-            vy[u] = t.y;
+        for (int par = 0; par < 2; par++) {
+            const double2* src = A + ((long)par * RT + rc) * S * 32;
+            double vx[KC], vy[KC];
+#pragma unroll
+            for (int u = 0; u < KC; u++) {
+                const int i = min(tid + u * 256, cnt - 1);
+                const double2 t = src[i];
+                vx[u] = t.x;
+                vy[u] = t.y;
+            }
+#pragma unroll
+            for (int u = 0; u < KC; u++)
+                if (tid + u * 256 < cnt)
+                    ((double2*)tvA)[par * kTvChunk * S * 32 + tid + u * 256] = make_double2(vx[u], vy[u]);
         }
-#pragma unroll
-        for (int u = 0; u < KC; u++)
-            if (tid + u * 256 < cnt) ((double2*)tvA)[tid + u * 256] = make_double2(vx[u], vy[u]);
         __syncthreads();
-        // tiles beyond RT hold stale values; their rows are >= nz and are not stored
-        rpcf_v4d acc[kTvChunk];
+        // tiles beyond RT hold stale values; their rows are > N/2 and are not stored
+        rpcf_v4d ae[kTvChunk], ao[kTvChunk];
 #pragma unroll
-        for (int t = 0; t < kTvChunk; t++) acc[t] = rpcf_v4d{0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < kTvChunk; t++) ae[t] = ao[t] = rpcf_v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < S; s++)
 #pragma unroll
-            for (int t = 0; t < kTvChunk; t++)
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tvA[(t * S + s) * 64 + lane], b[s], acc[t], 0, 0, 0);
+            for (int t = 0; t < kTvChunk; t++) {
+                ae[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tvA[(t * S + s) * 64 + lane], be[s], ae[t], 0, 0, 0);
+                ao[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(tvA[((kTvChunk + t) * S + s) * 64 + lane], bo[s], ao[t],
+                                                             0, 0, 0);
+            }
         if (okp) {
 #pragma unroll
             for (int t = 0; t < kTvChunk; t++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int row = 16 * (rc + t) + kq + 4 * r;
-                    if (row < n) Wm[2 * row] = acc[t][r];
+                    if (row < rows) {
+                        Wm[2 * row] = ae[t][r] + ao[t][r];
+                        if (N - row != row) Wm[2 * (N - row)] = ae[t][r] - ao[t][r];
+                    }
                 }
         }
     }

@@ -173,7 +173,7 @@ struct SlotSizes {
 SlotSizes slot_sizes(const Dims& d, int precision) {
     SlotSizes s;
     const long P = d.P, nz = d.nz, zld = (d.hnx + 7) & ~7;
-    const long RT = (nz + 15) / 16, S = (nz + 3) / 4;
+    const long RT = (d.N / 2 + 16) / 16, S = (d.N / 2 + 4) / 4;  // rows and columns per parity: N/2 + 1
     s.F = 6 * P * nz;
     s.S = P * nz;
     s.NL = 4 * P * nz;
@@ -183,7 +183,7 @@ SlotSizes slot_sizes(const Dims& d, int precision) {
     s.Wl = 5 * P * nz;
     s.M = 2L * d.nslot * nz * nz;
     s.MH = precision ? 2L * d.nslot * nz * (d.N / 2 + 1) : 0;  // nz rows x (N/2 + 1) pair columns
-    s.Tf = 5 * RT * S * 64;
+    s.Tf = 5 * 2 * RT * S * 64;
     s.Q = nz * nz;
     s.U = nz;
     s.Pz = d.pz;
@@ -531,12 +531,12 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
     int rc = launch(
         c, KID_LIN_TV,
         [&] {
-            const size_t lds = sizeof(double) * kTvChunk * c->tv_s * 64;
+            const size_t lds = sizeof(double) * 2 * kTvChunk * c->tv_s * 64;
             dim3 g((d.P + kTvBlockPencils - 1) / kTvBlockPencils, 5 * n);
             switch (2 * d.N) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
-        k_lin_tv_mfma<(L / 2 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, \
+        k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, \
                                                                    c->sTf, c->tv_rt, ids);                      \
         break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
@@ -691,8 +691,8 @@ int init_ctx(rpcf_ctx* c) {
     c->sF = ss.F; c->sS = ss.S; c->sNL = ss.NL; c->sY = ss.Y; c->sW = ss.W; c->sZ = ss.Z;
     c->sWl = ss.Wl; c->sM = ss.M; c->sMH = ss.MH; c->sTf = ss.Tf; c->sQ = ss.Q; c->sU = ss.U; c->sPz = ss.Pz;
     c->zld = (d.hnx + 7) & ~7;
-    c->tv_rt = (d.nz + 15) / 16;
-    c->tv_s = (d.nz + 3) / 4;
+    c->tv_rt = (d.N / 2 + 16) / 16;
+    c->tv_s = (d.N / 2 + 4) / 4;
     if ((rc = dalloc(c, &c->F, C * ss.F)) || (rc = dalloc(c, &c->SV, C * ss.S)) || (rc = dalloc(c, &c->SE, C * ss.S)) ||
         (rc = dalloc(c, &c->NL, C * ss.NL)) || (rc = dalloc(c, &c->Y, C * ss.Y)) || (rc = dalloc(c, &c->W, C * ss.W)) ||
         (rc = dalloc(c, &c->Z2, C * ss.Z)) || (rc = dalloc(c, &c->WL, C * ss.Wl)) ||
@@ -841,7 +841,7 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_span_fwd<L>, lim);    \
     allow_lds(k_lamb_phys<L>, lim);   \
     allow_lds(k_p2s_nl<L>, lim);      \
-    allow_lds(k_lin_tv_mfma<(L / 2 + 4) / 4>, lim);      \
+    allow_lds(k_lin_tv_mfma<(L / 4 + 4) / 4>, lim);      \
     allow_lds(k_cheb_s2p<1, L, float>, lim);              \
     allow_lds(k_span_inv<L, float>, lim);                 \
     allow_lds(k_span_fwd<L, float>, lim);                 \
@@ -927,13 +927,18 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
         const int RT = c->tv_rt, S = c->tv_s;
         const std::vector<double>* ops[5] = {&t0, &t2, &q4, &t0, &q2};  // op[j*nz + i] = T[i][j]
         std::vector<double> tf((size_t)c->sTf, 0.0);
+        // rows i <= N/2 against the even (par 0, j = 2k) and odd (par 1, j = 2k + 1) columns;
+        // the kernel mirrors rows N - i by the parity T[N-i][j] = (-1)^j T[i][j]
+        const size_t rows = d.N / 2 + 1;
         for (int m = 0; m < 5; m++)
-            for (int rt = 0; rt < RT; rt++)
-                for (int sk = 0; sk < S; sk++)
-                    for (int l = 0; l < 64; l++) {
-                        const size_t i = 16 * rt + (l & 15), k = 4 * sk + (l >> 4);
-                        if (i < nz && k < nz) tf[(((size_t)m * RT + rt) * S + sk) * 64 + l] = (*ops[m])[k * nz + i];
-                    }
+            for (int par = 0; par < 2; par++)
+                for (int rt = 0; rt < RT; rt++)
+                    for (int sk = 0; sk < S; sk++)
+                        for (int l = 0; l < 64; l++) {
+                            const size_t i = 16 * rt + (l & 15), j = 2 * (4 * sk + (l >> 4)) + par;
+                            if (i < rows && j < nz)
+                                tf[((((size_t)m * 2 + par) * RT + rt) * S + sk) * 64 + l] = (*ops[m])[j * nz + i];
+                        }
         RCCHK(copy_sync(c, c->TF + slot * c->sTf, tf.data(), sizeof(double) * tf.size(), hipMemcpyHostToDevice));
     }
     RCCHK(copy_sync(c, c->MP + slot, &sh.mp, sizeof(MemberParams), hipMemcpyHostToDevice));
