@@ -475,15 +475,18 @@ __device__ __forceinline__ void gemv_acc(double c, double2 u0, double2 u1, doubl
     a[2] = fma(c, u1.x, a[2]);
     a[3] = fma(c, u1.y, a[3]);
 }
+// Columns j < j0 are skipped (the caller's vectors vanish there: the boundary rows of the
+// right-hand sides).  The last, partial batch of B columns loads clamped addresses and
+// masks the vector by a multiply, so that its loads stay unconditional and in flight together.
 template <typename MT>
 __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const double2* x0, const double2* x1, int n,
-                                             double2* part, int tid, int nthr, double2* y0, double2* y1) {
+                                             double2* part, int tid, int nthr, double2* y0, double2* y1, int j0 = 0) {
     constexpr int B = 8;
     const int G = nthr / n;
     const int g = tid / n, r = tid - g * n;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     if (g < G) {
-        int j = g;
+        int j = j0 + g;
         for (; j + G * (B - 1) < n; j += G * B) {
             MT m[B];
 #pragma unroll
@@ -491,7 +494,18 @@ __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const dou
 #pragma unroll
             for (int u = 0; u < B; u++) gemv_acc(m[u], x0[j + G * u], x1[j + G * u], a);
         }
-        for (; j < n; j += G) gemv_acc(M[(long)j * n + r], x0[j], x1[j], a);
+        if (j < n) {
+            MT m[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) m[u] = M[(long)min(j + G * u, n - 1) * n + r];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int jj = min(j + G * u, n - 1);
+                const double k = j + G * u < n ? 1.0 : 0.0;
+                const double2 u0 = x0[jj], u1 = x1[jj];
+                gemv_acc(m[u], make_double2(u0.x * k, u0.y * k), make_double2(u1.x * k, u1.y * k), a);
+            }
+        }
         part[tid] = make_double2(a[0], a[1]);
         part[kSolveThreads + tid] = make_double2(a[2], a[3]);
     }
@@ -508,10 +522,10 @@ __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const dou
 }
 
 // The same with a rows x cols matrix (column-major, M[c*rows + r]) whose rows r < split
-// take the vectors xa and the others xb.
+// take the vectors xa and the others xb; columns from j0.
 __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const double2* xa0, const double2* xa1,
                                          const double2* xb0, const double2* xb1, int rows, int cols, int split,
-                                         double2* part, int tid, int nthr, double2* y0, double2* y1) {
+                                         double2* part, int tid, int nthr, double2* y0, double2* y1, int j0) {
     constexpr int B = 8;
     const int G = nthr / rows;
     const int g = tid / rows, r = tid - g * rows;
@@ -519,7 +533,7 @@ __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const do
     const double2* x1 = r < split ? xa1 : xb1;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     if (g < G) {
-        int j = g;
+        int j = j0 + g;
         for (; j + G * (B - 1) < cols; j += G * B) {
             double2 m[B];
 #pragma unroll
@@ -527,7 +541,18 @@ __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const do
 #pragma unroll
             for (int u = 0; u < B; u++) gemv_acc(m[u], x0[j + G * u], x1[j + G * u], a);
         }
-        for (; j < cols; j += G) gemv_acc(M[(long)j * rows + r], x0[j], x1[j], a);
+        if (j < cols) {
+            double2 m[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) m[u] = M[(long)min(j + G * u, cols - 1) * rows + r];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int jj = min(j + G * u, cols - 1);
+                const double k = j + G * u < cols ? 1.0 : 0.0;
+                const double2 u0 = x0[jj], u1 = x1[jj];
+                gemv_acc(m[u], make_double2(u0.x * k, u0.y * k), make_double2(u1.x * k, u1.y * k), a);
+            }
+        }
         part[tid] = make_double2(a[0], a[1]);
         part[kSolveThreads + tid] = make_double2(a[2], a[3]);
     }
@@ -580,16 +605,62 @@ __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __res
     }
 }
 
+// y_q = T0 x_q for the two vectors of a slot from the rows p <= N/2 of T0 (column-major,
+// T0t[j*nz + p] = T_j(z_p)) and the reflection parity T_j(z_{N-p}) = (-1)^j T_j(z_p):
+// E_p, O_p = the sums over the even and odd columns, y[p] = E_p + O_p, y[N-p] = E_p - O_p.
+// G (even) groups of N/2+1 threads; group g takes the columns j = g, g + G, .., all of the
+// parity of g.  The results land in y0[], y1[] (LDS, nz entries each).  Every thread of the
+// block calls it (it synchronises).
+__device__ __forceinline__ void gemv_t0_reflect(const double* __restrict__ T0t, const double2* x0, const double2* x1,
+                                                int n, int N, double2* part, int tid, int nthr, double2* y0,
+                                                double2* y1) {
+    constexpr int B = 8;
+    const int h = N / 2 + 1, G = (nthr / h) & ~1;
+    const int g = tid / h, r = tid - g * h;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g < G) {
+        int j = g;
+        for (; j + G * (B - 1) < n; j += G * B) {
+            double m[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) m[u] = T0t[(long)(j + G * u) * n + r];
+#pragma unroll
+            for (int u = 0; u < B; u++) gemv_acc(m[u], x0[j + G * u], x1[j + G * u], a);
+        }
+        for (; j < n; j += G) gemv_acc(T0t[(long)j * n + r], x0[j], x1[j], a);
+        part[tid] = make_double2(a[0], a[1]);
+        part[kSolveThreads + tid] = make_double2(a[2], a[3]);
+    }
+    __syncthreads();
+    if (tid < h) {
+        double2 e0 = part[tid], e1 = part[kSolveThreads + tid];
+        double2 o0 = part[h + tid], o1 = part[kSolveThreads + h + tid];
+        for (int q = 2; q < G; q += 2) {
+            e0 = zadd(e0, part[q * h + tid]);
+            e1 = zadd(e1, part[kSolveThreads + q * h + tid]);
+            o0 = zadd(o0, part[(q + 1) * h + tid]);
+            o1 = zadd(o1, part[kSolveThreads + (q + 1) * h + tid]);
+        }
+        y0[tid] = zadd(e0, o0);
+        y1[tid] = zadd(e1, o1);
+        if (N - tid != tid) {
+            y0[N - tid] = zsub(e0, o0);
+            y1[N - tid] = zsub(e1, o1);
+        }
+    }
+}
+
 // x = M b for the two right-hand sides rhs[0..1][nz] of a slot: with the full inverse M, or,
 // for a member with an even base profile, with its parity blocks H (k_parity_blocks): the
 // combined sides b_i + s b_i' and b_i - s b_i' per row pair, then one nz x np GEMV whose even
-// rows take the first and odd rows the second.  Thread r < nz returns x at coefficient
-// *jrow.  Every thread of the block calls it (it synchronises).
+// rows take the first and odd rows the second.  b vanishes on the rows below j0 (an even
+// count: whole row pairs), whose columns are skipped.  Thread r < nz returns x at
+// coefficient *jrow.  Every thread of the block calls it (it synchronises).
 __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const int4* pairs, int np, int parity,
                                            double2 (*rhs)[256], double2 (*bc)[2][128], double2* part, int n, int N,
-                                           int tid, int nthr, double2* y0, double2* y1, int* jrow) {
+                                           int tid, int nthr, double2* y0, double2* y1, int* jrow, int j0) {
     if (!parity) {
-        gemv_grouped(M, rhs[0], rhs[1], n, part, tid, nthr, y0, y1);
+        gemv_grouped(M, rhs[0], rhs[1], n, part, tid, nthr, y0, y1, j0);
         *jrow = tid;
         return;
     }
@@ -609,7 +680,7 @@ __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, 
     }
     __syncthreads();
     const int ne = N / 2 + 1;
-    gemv_sel(Hm, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1);
+    gemv_sel(Hm, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
     *jrow = tid < ne ? 2 * tid : 2 * (tid - ne) + 1;
 }
 
@@ -635,7 +706,7 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
                                                    const double* __restrict__ dU, const double* __restrict__ ddU, long sU,
                                                    const double* __restrict__ qw) {
     __shared__ double2 rhs[2][256], vnew[2][256], part[2][kSolveThreads];
-    __shared__ double2 bc[2][2][128];  // parity form: [vector][b_i + s b_i', b_i - s b_i'][pair]
+    __shared__ double2 bc[2][2][128];  // parity form: [vector][b_i + s b_i', b_i - s b_i'][pair]; T0 v^{n+1}
     __shared__ double2 s_f;
     LinState st;
     LinParams lp;
@@ -694,7 +765,7 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
             }
             __syncthreads();
             double2 x = make_double2(0.0, 0.0), uh = make_double2(0.0, 0.0);
-            gemv_grouped(e ? Mo : Mv, rhs[0], rhs[1], n, &part[0][0], r, blockDim.x, &x, &uh);
+            gemv_grouped(e ? Mo : Mv, rhs[0], rhs[1], n, &part[0][0], r, blockDim.x, &x, &uh, 2);
             if (r < n) {
                 vnew[0][r] = x;
                 vnew[1][r] = uh;
@@ -752,23 +823,19 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
         double2 y0, y1;
         int j;
         solve_slot(Mv, st.minvh + (long)slot * nh, pairs, np, st.parity, rhs, bc, &part[0][0], n, d.N, r,
-                   blockDim.x, &y0, &y1, &j);
+                   blockDim.x, &y0, &y1, &j, 4);
         if (r < n) vnew[0][j] = y0, vnew[1][j] = y1;
     }
     __syncthreads();
-    {
-        // T0 v^{n+1}, row p -> tv[.][p]
-        double2 y0, y1;
-        gemv_grouped(T0t, vnew[0], vnew[1], n, &part[0][0], r, blockDim.x, &y0, &y1);
-        __syncthreads();
-        if (r < n) part[0][r] = y0, part[1][r] = y1;
-    }
+    // T0 v^{n+1} at the points p -> tv0[p], tv1[p] (in bc, free until the eta solve)
+    double2 *tv0 = &bc[0][0][0], *tv1 = &bc[1][0][0];
+    gemv_t0_reflect(T0t, vnew[0], vnew[1], n, d.N, &part[0][0], r, blockDim.x, tv0, tv1);
     __syncthreads();
     if (r < n) {
         double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
         if (r >= 2) {
             const int p = r - 1;
-            const double w0r = part[0][p].x, w0i = part[0][p].y, w1r = part[1][p].x, w1i = part[1][p].y;
+            const double w0r = tv0[p].x, w0i = tv0[p].y, w1r = tv1[p].x, w1i = tv1[p].y;
             const double c0k = (1 - kmn * dt * 0.5 / Re), ci = -alpha * dt * 0.5 * U[p];
             const double e0 = -0.5 * beta0 * dU[p] * dt, e1 = -0.5 * beta1 * dU[p] * dt;
             const double2 s0 = W3[po0 + p], s2 = W4[po0 + p], sv0 = W0[po0 + p];
@@ -786,7 +853,7 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
     double2 e0, e1;
     int je;
     solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, pairs + np, np, st.parity, rhs, bc, &part[0][0], n, d.N,
-               r, blockDim.x, &e0, &e1, &je);
+               r, blockDim.x, &e0, &e1, &je, 2);
     if (r < n) {
         st.se[po0 + je] = e0;
         st.sv[po0 + r] = vnew[0][r];
