@@ -1021,7 +1021,9 @@ __global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1
     const double factor = (1.0 / (d.pz - 1));
     const int total = nb * nz, nsamp = nb * pz;
     const cplx<RI>* in0 = Z1 + (long)kx * d.pz * ny + ky0;
-    double2 smp[KL];
+    // samples are held as loaded (FP32 in the mixed form) and widened where they are used, so
+    // that the prefetch of the next component does not wait for its loads
+    cplx<RI> smp[KL];
     auto load = [&](int g) {
         const cplx<RI>* in = in0 + g * zstride;
 #pragma unroll
@@ -1029,7 +1031,7 @@ __global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1
             const int i = tid + u * nthr;
             if (i < nsamp) {
                 const int row = i / nb, b = i - row * nb;
-                smp[u] = to_d2(in[(long)row * ny + b]);
+                smp[u] = in[(long)row * ny + b];
             }
         }
     };
@@ -1043,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1
             const int i = tid + u * nthr;
             if (i < nsamp) {
                 const int row = i / nb, b = i - row * nb, m = row >> 1;
-                const double2 v = smp[u];
+                const double2 v = to_d2(smp[u]);
                 if ((row & 1) == 0) {
                     double2* sq = fb + b * ld;
                     sq[m] = v;
