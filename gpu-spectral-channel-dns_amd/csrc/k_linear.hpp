@@ -863,9 +863,3 @@ __global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restr
         }
     }
 }
-
-// AB2 history swap of every listed member (source/rhs.cpp:107-108): the buffer that held
-// N^{n-1} becomes the current one and receives the new N.
-__global__ void k_flip(int* __restrict__ phase, const int* __restrict__ ids, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) phase[ids[i]] ^= 1;
-}

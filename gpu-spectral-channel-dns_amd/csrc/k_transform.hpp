@@ -565,11 +565,16 @@ __global__ __launch_bounds__(256) void k_x_r2c(const double* __restrict__ phys, 
 template <int FN = 0, typename R = double>  // R: precision of Z2, Z1 and the FFT
 __global__ __launch_bounds__(256) void k_span_fwd(const cplx<R>* __restrict__ Z2, long zstride, long sZ, int zld,
                                                   cplx<R>* __restrict__ Z1, long ostride, long sY, Dims d, int planes,
-                                                  FftPlan pl, int TK, int ld, const int* __restrict__ ids) {
+                                                  FftPlan pl, int TK, int ld, const int* __restrict__ ids,
+                                                  int* __restrict__ phase = nullptr) {
     using C2 = cplx<R>;
     RPCF_LDS(C2);
     const int g = blockIdx.z % 3, q = blockIdx.y, kx0 = blockIdx.x * TK;
     const long slot = ids[blockIdx.z / 3];
+    // the AB2 history swap of the step (source/rhs.cpp:107-108), by one thread per member:
+    // nothing in this kernel reads the phase, and the next kernel (the p2s) writes the new N
+    // into the buffers the flipped phase names current
+    if (phase && blockIdx.x == 0 && blockIdx.y == 0 && g == 0 && threadIdx.x == 0) phase[slot] ^= 1;
     Z2 += slot * sZ;
     Z1 += slot * sY;
     const int nk_rt = min(TK, d.hnx - kx0);

@@ -468,20 +468,19 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
             c, KID_SPAN_FWD,
             [&] {
                 switch (d.my) {
-#define RPCF_CASE(L) case L: k_span_fwd<L, R><<<g, kFftThreads, lds, c->stream>>>(Z2, zS, c->sZ, c->zld, Y, yS, c->sY, d, d.pz, ply, TK, ld, ids); break;
+#define RPCF_CASE(L) case L: k_span_fwd<L, R><<<g, kFftThreads, lds, c->stream>>>(Z2, zS, c->sZ, c->zld, Y, yS, c->sY, d, d.pz, ply, TK, ld, ids, c->PH); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
                         if constexpr (f64)
                             k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(Z2, zS, c->sZ, c->zld, Y, yS, c->sY, d,
-                                                                           d.pz, ply, TK, ld, ids);
+                                                                           d.pz, ply, TK, ld, ids, c->PH);
                 }
             },
             cap);
         if (rc) return rc;
     }
-    // AB2 history swap (source/rhs.cpp:107-108): the new N lands in the old "previous" buffers
-    k_flip<<<1, 64, 0, c->stream>>>(c->PH, ids, n);
+    // the AB2 history swap ran inside k_span_fwd: the new N lands in the old "previous" buffers
     if (c->TB_nl2) {
         const int TB = c->TB_nl2, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
