@@ -189,3 +189,23 @@ def test_ragged_tiles_step_parity():
     a, b = g.get_state(), o.get_state()
     assert rel(a[0], b[0]) < 1e-10 and rel(a[1], b[1]) < 1e-10
     g.close(), o.close()
+
+
+def test_fallback_lengths_step_parity():
+    """lengths without compiled plans (60x30x80: 2N = 40, mx = 60, my = 30) take the runtime
+    FFT plans, the generic velocity, Lamb and p2s kernels and the operator products off the
+    matrix cores: one step to 1e-12, ten to 1e-10"""
+    prm = rpcf.make_params(**ODD)
+    g, o = pair(prm)
+    v0, e0 = rpcf.seed_state(o.dll, prm, 5, 0.05, 4, 8, 0.0)
+    for sim in (g, o):
+        sim.set_state(v0, e0)
+        sim.bootstrap()
+    a, b = g.get_state(), o.get_state()
+    print("ODD 1 step", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
+    g.step(9), o.step(9)
+    a, b = g.get_state(), o.get_state()
+    print("ODD 10 steps", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-10 and rel(a[1], b[1]) < 1e-10
+    g.close(), o.close()
