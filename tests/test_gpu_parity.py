@@ -209,3 +209,27 @@ def test_fallback_lengths_step_parity():
     print("ODD 10 steps", rel(a[0], b[0]), rel(a[1], b[1]))
     assert rel(a[0], b[0]) < 1e-10 and rel(a[1], b[1]) < 1e-10
     g.close(), o.close()
+
+
+# the longest compiled Chebyshev length, 2N = 384 (nz = 193, pz = 385: three-stage FFT plans,
+# the largest register tiles of the Chebyshev kernels and 25 k-steps of the operator products)
+DEEP = dict(mx=24, my=24, mz=768, n_pi_x=2.0, n_pi_y=1.0, Re=2000.0, Ro=0.0, dt=0.002,
+            u_profile=(1.0, 0.0, -1.0), flow_control=1)
+
+
+def test_longest_compiled_chebyshev_step_parity():
+    """one step to 1e-12 and five to 1e-11 at nz = 193"""
+    prm = rpcf.make_params(**DEEP)
+    g, o = pair(prm)
+    v0, e0 = rpcf.seed_state(o.dll, prm, 7, 0.05, 8, 8, 0.0)
+    for sim in (g, o):
+        sim.set_state(v0, e0)
+        sim.bootstrap()
+    a, b = g.get_state(), o.get_state()
+    print("DEEP 1 step", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
+    g.step(4), o.step(4)
+    a, b = g.get_state(), o.get_state()
+    print("DEEP 5 steps", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-11 and rel(a[1], b[1]) < 1e-11
+    g.close(), o.close()
