@@ -19,6 +19,9 @@ CASES = {
     "C1": (params("C1"), IC["C1"], 100),
     "C2": (rpcf.make_params(mx=384, my=384, mz=384, n_pi_x=4.0, n_pi_y=2.0, Re=1300.0, Ro=0.1, dt=0.002,
                             u_profile=(0.0, 1.0, 0.0)), (0.05, 16, 24, 5.0 / 6.0), 200),
+    # the longest compiled Chebyshev length (nz = 193) with an even profile: 97 parity pairs
+    "DEEP": (rpcf.make_params(mx=24, my=24, mz=768, n_pi_x=2.0, n_pi_y=1.0, Re=2000.0, Ro=0.0, dt=0.002,
+                              u_profile=(1.0, 0.0, -1.0), flow_control=1), (0.05, 8, 8, 0.0), 50),
 }
 
 
