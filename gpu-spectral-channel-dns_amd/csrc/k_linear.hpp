@@ -376,10 +376,13 @@ typedef double rpcf_v4d __attribute__((ext_vector_type(4)));
 constexpr int kTvChunk = 3;
 constexpr int kTvBlockPencils = 32;
 
-// three waves per SIMD (168 registers, no accumulation registers): the B-fragment loads of
-// more waves in flight; two waves (136 + 48) took 76 us against 68 at 128x129x128
+// three waves per SIMD (168 registers, no accumulation registers) up to S = 17: the
+// B-fragment loads of more waves in flight; two waves (136 + 48) took 76 us against 68 at
+// 128x129x128.  Longer Chebyshev lengths (S = 25, 33) hold 4 S B-fragment registers and
+// would spill at three waves, so they run two.
+constexpr int tv_waves(int s) { return s <= 17 ? 3 : 2; }
 template <int S>  // k-steps per parity: S >= ceil((N/2 + 1)/4)
-__global__ __launch_bounds__(256, 3) void k_lin_tv_mfma(const double2* __restrict__ sv, const double2* __restrict__ se,
+__global__ __launch_bounds__(256, tv_waves(S)) void k_lin_tv_mfma(const double2* __restrict__ sv, const double2* __restrict__ se,
                                                      long sS, double2* __restrict__ W, long sWl, Dims d,
                                                      const double* __restrict__ Tf, long sTf, int RT,
                                                      const int* __restrict__ ids) {

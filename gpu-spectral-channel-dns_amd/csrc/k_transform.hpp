@@ -991,9 +991,11 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const cplx<RI>* __restrict__ Z1,
 // Results are bit-identical to k_p2s_nl (same butterflies, same operation order).
 // LDS: [2*TB][ld] sequences, 256 scan totals, FN FFT twiddles, nz p2s twiddles.
 // -------------------------------------------------------------------------------------
-// three waves per SIMD (168 registers instead of 172 at two waves)
+// three waves per SIMD (168 registers instead of 172 at two waves) up to 2N = 256; the
+// longer lengths keep more samples and coefficients per thread and run two waves (no spills)
+constexpr int p2s_waves(int fn) { return fn <= 256 ? 3 : 2; }
 template <int FN, typename RI = double>
-__global__ __launch_bounds__(256, 3) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1, long zstride, long sY,
+__global__ __launch_bounds__(256, p2s_waves(FN)) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1, long zstride, long sY,
                                                  double2* __restrict__ NL, long sNL, Dims d, FftPlan plN,
                                                  const double2* __restrict__ tw4, int TB, int ld, Members mb) {
     static_assert(FN > 0, "compiled lengths only");

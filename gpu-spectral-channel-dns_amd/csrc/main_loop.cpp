@@ -4,13 +4,15 @@
 //   rpcf_run [--lib PATH] [PARAMETER_FILE]
 //
 // PARAMETER_FILE (default ../input/parameter.txt, as the reference) holds the reference's
-// six whitespace-separated lines (RPCF_Paras::read_para, source/operation.cpp:152-175):
+// six whitespace-separated lines (RPCF_Paras::read_para, source/operation.cpp:115-138):
 //   mx my mz / n_pi_x n_pi_y / Re Ro dt /
 //   start_type start_step end_step save_internal save_recovery_internal /
 //   output_prefix / recovery_prefix
 // and optionally two extension lines (absent: reference behaviour):
 //   c0 c1 c2 flow_control precision    base flow U = c0 + c1 y + c2 y^2, flow-rate control, 0 FP64 / 1 mixed
-//   seed amplitude kmax degmax kexp    seeded initial field (rpcf_seed_state); without
+//   seed amplitude kmax degmax kexp [peak_bound]
+//                                      seeded initial field (rpcf_seed_state; peak_bound
+//                                      0, the default: coefficients x amplitude); without
 //                                      this line start_type 0 takes the reference's
 //                                      analytic field (init_flow_kernel)
 //
@@ -56,7 +58,7 @@ struct Api {
     int (*set_nonlinear)(rpcf_ctx*, int, const double*, const double*, long);
     int (*step)(rpcf_ctx*, int);
     int (*bootstrap)(rpcf_ctx*);
-    int (*seed_state)(const rpcf_params*, uint64_t, double, int, int, double, double*, double*, long);
+    int (*seed_state)(const rpcf_params*, uint64_t, double, int, int, double, int, double*, double*, long);
     int (*get_field)(rpcf_ctx*, int, double*, long);
     int (*phys_to_spec)(rpcf_ctx*, const double*, double*, long, int);
     int (*spec_to_phys)(rpcf_ctx*, const double*, double*, long, int);
@@ -91,10 +93,11 @@ struct Config {
     bool seeded = false;  // line 8 present: seeded field; absent: the reference's analytic field
     uint64_t seed = 0;
     double amplitude = 1e-2, kexp = 0.0;
+    int peak_bound = 0;
     int kmax = 4, degmax = 8;
 };
 
-// RPCF_Paras::read_para (source/operation.cpp:152-175) plus the two extension lines
+// RPCF_Paras::read_para (source/operation.cpp:115-138) plus the two extension lines
 bool read_para(const std::string& file, Config* c) {
     std::ifstream in(file);
     if (!in.is_open()) {
@@ -133,9 +136,11 @@ bool read_para(const std::string& file, Config* c) {
         std::istringstream e(lines[7]);
         e >> c->seed >> c->amplitude >> c->kmax >> c->degmax >> c->kexp;
         if (e.fail()) {
-            std::cerr << "rpcf_run: line 8 must be: seed amplitude kmax degmax kexp" << std::endl;
+            std::cerr << "rpcf_run: line 8 must be: seed amplitude kmax degmax kexp [peak_bound]" << std::endl;
             return false;
         }
+        int pb = 0;
+        if (e >> pb) c->peak_bound = pb;
         c->seeded = true;
     }
     return true;
@@ -174,7 +179,7 @@ struct InitProfile {
 bool reference_initial_state(const Api& a, rpcf_ctx* ctx, const rpcf_params& p, int nz, size_t P,
                              std::vector<double>* v, std::vector<double>* e) {
     const double pi = 3.14159265358979323846;
-    // lx = 2 pi aphi, ly = 2 pi beta with aphi = n_pi_x/2, beta = n_pi_y/2 (source/data.cpp:5-6)
+    // lx = 2 pi aphi, ly = 2 pi beta with aphi = n_pi_x/2, beta = n_pi_y/2 (source/data.cpp:5-6; include/data.h:321-322)
     const InitProfile prof{pi * p.n_pi_x, pi * p.n_pi_y};
     const size_t plane = (size_t)p.mx * p.my;
     std::vector<double> fw(plane * nz, 0.0), fo(plane * nz, 0.0);
@@ -345,7 +350,8 @@ int main(int argc, char** argv) {
         const size_t n = 2 * z.P * z.nz;
         std::vector<double> v(n), e(n);
         if (cfg.seeded) {
-            if (api.seed_state(&cfg.prm, cfg.seed, cfg.amplitude, cfg.kmax, cfg.degmax, cfg.kexp, v.data(), e.data(),
+            if (api.seed_state(&cfg.prm, cfg.seed, cfg.amplitude, cfg.kmax, cfg.degmax, cfg.kexp, cfg.peak_bound,
+                               v.data(), e.data(),
                                z.nz))
                 return fail("seeded initial field");
         } else if (!reference_initial_state(api, ctx, cfg.prm, z.nz, z.P, &v, &e)) {

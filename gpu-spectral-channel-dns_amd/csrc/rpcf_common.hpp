@@ -2,7 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-// Grid sizes in the reference's naming (include/data.h:189-334, source/data.cpp:20-32):
+// Grid sizes in the reference's naming (include/data.h:189-334, source/data.cpp:3-14):
 // mx, my padded x / span lengths; nx = mx/3*2, ny = my/3*2 kept modes; nz = mz/4+1
 // Chebyshev coefficients, N = nz-1; pz = mz/2+1 = 2N+1 padded wall-normal points;
 // hnx = nx/2+1 kx pencils (the last one is the x-Nyquist, zeroed on the way to physical

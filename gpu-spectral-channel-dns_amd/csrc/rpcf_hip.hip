@@ -201,12 +201,14 @@ hipEvent_t get_event(rpcf_ctx* c) {
     return e;
 }
 
-// Wait for the stream, then fold the recorded kernel times into the totals.
+// Wait for the stream, then fold the recorded kernel times into the totals.  The recorded
+// events go back to the pool whatever the outcome, so that a failed wait leaves no stale
+// times for a later call to add.
 int sync_ctx(rpcf_ctx* c) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const hipError_t e = hipStreamSynchronize(c->stream);
     for (auto& p : c->pending) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        if (e == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             c->t_ms[p.kid] += ms;
             c->t_cnt[p.kid] += 1;
         }
@@ -214,6 +216,7 @@ int sync_ctx(rpcf_ctx* c) {
         c->pool.push_back(p.b);
     }
     c->pending.clear();
+    if (e != hipSuccess) return fail(c, RPCF_EDEVICE, "hipStreamSynchronize: %s", hipGetErrorString(e));
     return RPCF_OK;
 }
 
@@ -228,7 +231,10 @@ int launch(rpcf_ctx* c, int kid, F&& fn, bool capturing = false) {
     }
     fn();
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(c, RPCF_EDEVICE, "launch %s: %s", kKernelNames[kid], hipGetErrorString(e));
+    if (e != hipSuccess) {
+        if (a) c->pool.push_back(a);
+        return fail(c, RPCF_EDEVICE, "launch %s: %s", kKernelNames[kid], hipGetErrorString(e));
+    }
     if (timed) {
         b = get_event(c);
         (void)hipEventRecord(b, c->stream);
@@ -1250,9 +1256,13 @@ int rpcf_bootstrap(rpcf_ctx* c) {
     return rc ? rc : rpcf_step(c, 1);
 }
 
-int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride, int padded) {
-    if (!c) return RPCF_EINVAL;
-    if (!spec || !phys) return fail(c, RPCF_EINVAL, "rpcf_spec_to_phys: null array");
+}  // extern "C"
+
+namespace {
+// The backward transform of one field on the context's stream: Chebyshev s2p (pencils f,
+// nz contiguous) -> planes y -> span C2C -> planes w -> x C2R -> ph; y and w are the
+// caller's intermediates of hnx*planes*ny and planes*my*nxh complex values.
+int enq_spec_to_phys(rpcf_ctx* c, const double2* f, double2* y, double2* w, double* ph, int padded) {
     const Dims d = c->d;
     const int planes = padded ? d.pz : d.nz;
     const FftPlan& plz = c->pl_zn;
@@ -1260,6 +1270,28 @@ int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride
     const int TB = pick_tile(16, L, c->cap_zn, c->lds_limit, [&](int t) { return sizeof(double2) * (size_t)t * ld; });
     const int RP1 = pick_tile(8, d.mx, c->cap_x, c->lds_limit, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_mx; });
     const double2* tw4 = (const double2*)c->pl_zp.tw;
+    dim3 g((d.ny + TB - 1) / TB, d.hnx, 1);
+    const size_t lds = sizeof(double2) * TB * ld;
+    if (padded) k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(f, 0, 0, y, 0, 0, d, plz, tw4, TB, ld, c->ids_zero);
+    else k_cheb_s2p<0><<<g, kFftThreads, lds, c->stream>>>(f, 0, 0, y, 0, 0, d, plz, tw4, TB, ld, c->ids_zero);
+    dim3 g2((d.nxh + c->TK_inv - 1) / c->TK_inv, planes, 1);
+    k_span_inv<0><<<g2, kFftThreads, sizeof(double2) * c->TK_inv * c->ld_my, c->stream>>>(
+        y, 0, 0, w, 0, 0, d, planes, c->pl_y, c->TK_inv, c->ld_my, c->ids_zero);
+    dim3 g3((d.my / 2 + RP1 - 1) / RP1, planes);
+    k_x_c2r<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(w, ph, d, c->pl_x, RP1, c->ld_mx);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RPCF_OK : fail(c, RPCF_EDEVICE, "spec_to_phys launch: %s", hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride, int padded) {
+    if (!c) return RPCF_EINVAL;
+    if (!spec || !phys) return fail(c, RPCF_EINVAL, "rpcf_spec_to_phys: null array");
+    const Dims d = c->d;
+    const int planes = padded ? d.pz : d.nz;
     double2 *f = nullptr, *y = nullptr, *w = nullptr;
     double* ph = nullptr;
     int rc = dalloc(c, &f, spec_elems(c));
@@ -1267,19 +1299,7 @@ int rpcf_spec_to_phys(rpcf_ctx* c, const double* spec, double* phys, long stride
     if (!rc) rc = dalloc(c, &w, (size_t)planes * d.my * d.nxh);
     if (!rc) rc = dalloc(c, &ph, (size_t)planes * d.my * d.mx);
     if (!rc) rc = upload_spec(c, f, spec, stride);
-    if (!rc) {
-        dim3 g((d.ny + TB - 1) / TB, d.hnx, 1);
-        const size_t lds = sizeof(double2) * TB * ld;
-        if (padded) k_cheb_s2p<1><<<g, kFftThreads, lds, c->stream>>>(f, 0, 0, y, 0, 0, d, plz, tw4, TB, ld, c->ids_zero);
-        else k_cheb_s2p<0><<<g, kFftThreads, lds, c->stream>>>(f, 0, 0, y, 0, 0, d, plz, tw4, TB, ld, c->ids_zero);
-        dim3 g2((d.nxh + c->TK_inv - 1) / c->TK_inv, planes, 1);
-        k_span_inv<0><<<g2, kFftThreads, sizeof(double2) * c->TK_inv * c->ld_my, c->stream>>>(
-            y, 0, 0, w, 0, 0, d, planes, c->pl_y, c->TK_inv, c->ld_my, c->ids_zero);
-        dim3 g3((d.my / 2 + RP1 - 1) / RP1, planes);
-        k_x_c2r<<<g3, kFftThreads, sizeof(double2) * RP1 * c->ld_mx, c->stream>>>(w, ph, d, c->pl_x, RP1, c->ld_mx);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) rc = fail(c, RPCF_EDEVICE, "spec_to_phys launch: %s", hipGetErrorString(e));
-    }
+    if (!rc) rc = enq_spec_to_phys(c, f, y, w, ph, padded);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, RPCF_EDEVICE, "spec_to_phys sync");
     if (!rc) rc = copy_sync(c, phys, ph, sizeof(double) * planes * d.my * d.mx, hipMemcpyDeviceToHost);
     for (void* p : {(void*)f, (void*)y, (void*)w, (void*)ph})
@@ -1384,7 +1404,7 @@ static double seed_velocity_bound(const rpcf_params* prm, const double* v, const
 }
 
 int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int kmax, int degmax, double kexp,
-                    double* v_hat, double* eta_hat, long stride) {
+                    int peak_bound, double* v_hat, double* eta_hat, long stride) {
     Dims d;
     if (check_sizes(prm, &d)) return RPCF_EINVAL;
     const int nz = d.nz, ny = d.ny, hnx = d.hnx, nx = d.nx;
@@ -1428,10 +1448,11 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
             }
         }
     }
-    // amplitude: the bound of the peak perturbation velocity
-    const double bound = seed_velocity_bound(prm, v_hat, eta_hat, stride, nz, ny, hnx);
+    // amplitude: the factor of the Gaussian coefficients, or (peak_bound) the bound of the
+    // peak perturbation velocity
+    const double bound = peak_bound ? seed_velocity_bound(prm, v_hat, eta_hat, stride, nz, ny, hnx) : 1.0;
     if (bound > 0.0) {
-        const double f = amplitude / bound;
+        const double f = peak_bound ? amplitude / bound : amplitude;
         for (size_t p = 0; p < P; p++)
             for (int k = 0; k < 2 * nz; k++) {
                 v_hat[2 * (size_t)stride * p + k] *= f;

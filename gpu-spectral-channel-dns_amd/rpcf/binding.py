@@ -97,7 +97,7 @@ def load(path):
         "rpcf_phys_to_spec": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_long, ctypes.c_int]),
         "rpcf_spec_to_phys": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_long, ctypes.c_int]),
         "rpcf_seed_state": (ctypes.c_int, [P, ctypes.c_uint64, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                           ctypes.c_double, _dp, _dp, ctypes.c_long]),
+                                           ctypes.c_double, ctypes.c_int, _dp, _dp, ctypes.c_long]),
         "rpcf_get_inverse": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     }
     ip = ctypes.POINTER(ctypes.c_int)
@@ -135,11 +135,13 @@ def sizes(dll, prm):
     return dict(nx=nx, ny=ny, nz=nz, pz=pz, hnx=hnx, P=hnx * ny)
 
 
-def seed_state(dll, prm, seed, amplitude, kmax, degmax, kexp=0.0):
+def seed_state(dll, prm, seed, amplitude, kmax, degmax, kexp=0.0, peak_bound=True):
+    """rpcf_seed_state: peak_bound=False multiplies the Gaussian coefficients by `amplitude`
+    (the stated recipe); True scales the field so that `amplitude` bounds its peak velocity"""
     s = sizes(dll, prm)
     v = np.zeros((s["P"], s["nz"]), np.complex128)
     e = np.zeros((s["P"], s["nz"]), np.complex128)
-    rc = dll.rpcf_seed_state(ctypes.byref(prm), seed, amplitude, kmax, degmax, kexp,
+    rc = dll.rpcf_seed_state(ctypes.byref(prm), seed, amplitude, kmax, degmax, kexp, int(bool(peak_bound)),
                              _ptr(v.view(np.float64)), _ptr(e.view(np.float64)), s["nz"])
     if rc != 0:
         raise RpcfError(f"rpcf_seed_state rc={rc}")

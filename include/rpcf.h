@@ -9,7 +9,7 @@
  * Data conventions (shared by both libraries and by every host array crossing the ABI)
  * ----------------------------------------------------------------------------------
  *  Sizes follow the reference's problem struct (include/data.h:189-334,
- *  source/data.cpp:3-32):  nx = mx/3*2, ny = my/3*2, nz = mz/4+1, pz = mz/2+1,
+ *  source/data.cpp:3-14):  nx = mx/3*2, ny = my/3*2, nz = mz/4+1, pz = mz/2+1,
  *  hnx = nx/2+1.  "x" is streamwise, reference "y" is spanwise, reference "z" is
  *  wall-normal (Chebyshev, points cos(pi*p/(nz-1))).
  *  Spectral arrays are interleaved complex double, one "pencil" per (kx,ky) with
@@ -24,7 +24,7 @@
  *
  * Errors: every entry point returns RPCF_OK (0) or a negative code; the message of
  * the last failure on a context is returned by rpcf_last_error.  Nothing exits the
- * process (the reference's cuCheck exits, source/operation.cpp:123-133).
+ * process (the reference's cuCheck exits, source/operation.cpp:86-96).
  * Threading: one context per host thread, or external synchronisation.
  */
 #ifndef RPCF_H
@@ -39,7 +39,7 @@ extern "C" {
 typedef struct rpcf_ctx rpcf_ctx;
 
 /* Numerical parameters.  mx..dt mirror RPCF_Numerical_Para (include/parameters.h:4-13)
- * and parameter.txt lines 1-3 (source/operation.cpp:152-175): Lx = pi*n_pi_x,
+ * and parameter.txt lines 1-3 (source/operation.cpp:115-138): Lx = pi*n_pi_x,
  * Lz(span) = pi*n_pi_y.  The remaining fields extend the reference:
  *   u_profile     base flow U(y) = c0 + c1*y + c2*y^2 replacing the hard-coded
  *                 U = (1+y)/2 of source/coefficient.cu:149-159 and
@@ -84,7 +84,7 @@ enum {
 
 /* Derived sizes of a parameter set; any pointer may be NULL.  Returns RPCF_EINVAL
  * for sizes the implementation cannot run.  Replaces problem::initVars
- * (source/data.cpp:20-49). */
+ * (source/data.cpp:3-31). */
 int rpcf_sizes(const rpcf_params* prm, int* nx, int* ny, int* nz, int* pz, int* hnx);
 
 /* Allocate a context: operator T matrices, base profile, per-mode Crank-Nicolson
@@ -150,14 +150,17 @@ int rpcf_get_inverse(rpcf_ctx* ctx, int eq, int pencil, double* out);
  * Chebyshev coefficients of degree <= degmax on pencils with kx <= kmax and
  * |signed ky| <= kmax (pencil (0,0) excluded), times (1+|k|)^(-kexp) with
  * |k| = sqrt(kx^2 + ky^2) in index units, multiplied by (1-y^2)^2 for v and (1-y^2)
- * for eta, Hermitian on kx = 0, Nyquist pencils zero; then scaled so that `amplitude`
- * bounds the peak perturbation velocity: for each of u, v, w the sum of the magnitudes of
- * its spectral coefficients (which bounds its maximum over the domain) is at most
- * `amplitude`.  Generator: splitmix64 + Box-Muller, draws in pencil order.  No
- * counterpart in the reference (its init_flow_kernel analytic field,
- * source/RPCFKernels.cu:221-356, is replaced). */
+ * for eta, Hermitian on kx = 0, Nyquist pencils zero.  peak_bound = 0: the coefficients
+ * are then multiplied by `amplitude` (the recipe as the benchmark configurations state
+ * it); peak_bound = 1: the field is scaled so that `amplitude` bounds the peak
+ * perturbation velocity: for each of u, v, w the sum of the magnitudes of its spectral
+ * coefficients (which bounds its maximum over the domain) is at most `amplitude` (for
+ * recipes whose literal field is too strong for the time step).  Generator: splitmix64 +
+ * Box-Muller, draws in pencil order.  No counterpart in the reference (its
+ * init_flow_kernel analytic field, source/RPCFKernels.cu:221-356, is replaced). */
 int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int kmax,
-                    int degmax, double kexp, double* v_hat, double* eta_hat, long pencil_stride);
+                    int degmax, double kexp, int peak_bound, double* v_hat, double* eta_hat,
+                    long pencil_stride);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every
  * launch (no counterpart in the reference, whose step timer is one cudaEvent pair,

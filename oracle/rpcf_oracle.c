@@ -23,6 +23,12 @@
  *     by a dense solve on small systems and by the physics checks in tests/.
  *
  * Parity status of each function is listed in the table at the top of DESIGN.md.
+ *
+ * rpcf_oracle_plant (exported, 0 by default) plants a known mistake for the tests that
+ * show a pin fails on it: 1 flips the sign of the Coriolis term of L_x, 2 that of L_z,
+ * 3 the sign of the lift-up coupling of the eta right-hand side, 4 the sign of the
+ * Squire advection term -i alpha U eta.  With 0 every factor is exactly 1.0, so the
+ * arithmetic is bit for bit the unplanted one.
  */
 #include "../include/rpcf.h"
 
@@ -159,6 +165,9 @@ static void fft_exec(const fftplan* pl, const cx* in, cx* out, int inv) {
 /* ------------------------------------------------------------------------------ */
 /* context                                                                          */
 /* ------------------------------------------------------------------------------ */
+int rpcf_oracle_plant = 0;
+static double plant_sign(int which) { return rpcf_oracle_plant == which ? -1.0 : 1.0; }
+
 struct rpcf_ctx {
     rpcf_params prm;
     int mx, my, mz, nx, ny, nz, pz, hnx, N, P;
@@ -185,7 +194,7 @@ static int set_err(rpcf_ctx* c, int code, const char* msg) {
     return code;
 }
 
-/* problem(RPCF_Paras) + problem::initVars: include/data.h:312-328, source/data.cpp:20-32 */
+/* problem(RPCF_Paras) + problem::initVars: include/data.h:312-328, source/data.cpp:3-14 */
 static int check_sizes(const rpcf_params* prm, int* nx, int* ny, int* nz, int* pz, int* hnx) {
     if (!prm) return RPCF_EINVAL;
     int mx = prm->mx, my = prm->my, mz = prm->mz;
@@ -410,7 +419,7 @@ int rpcf_create(const rpcf_params* prm, rpcf_ctx** out) {
     c->nx = nx; c->ny = ny; c->nz = nz; c->pz = pz; c->hnx = hnx; c->N = nz - 1;
     c->P = hnx * ny;
     const double PI = 4.0 * atan(1.0);
-    /* include/data.h:321-322, source/data.cpp:21-23 */
+    /* include/data.h:321-322, source/data.cpp:5-6 */
     c->aphi = prm->n_pi_x / 2;
     c->beta = prm->n_pi_y / 2;
     c->lx = 2 * PI * c->aphi;
@@ -668,16 +677,16 @@ static void ddz(const cx* u, cx* out, int n) {
 /* computeLambDevice: source/nonlinear.cu:409-428, base flow at padded plane q */
 static void lamb(const rpcf_ctx* c, double** ph) {
     const int pz = c->pz, my = c->my, mx = c->mx;
-    const double Ro = c->Ro;
+    const double Ro = c->Ro, Rx = plant_sign(1) * Ro, Rz = plant_sign(2) * Ro;
 #pragma omp parallel for schedule(static)
     for (int q = 0; q < pz; q++) {
         double meanU = c->meanU_pad[q];
         for (size_t i = (size_t)q * my * mx; i < (size_t)(q + 1) * my * mx; i++) {
             double u = ph[0][i], v = ph[1][i], w = ph[2][i];
             double ox = ph[3][i], oy = ph[4][i], oz = ph[5][i];
-            double lx = oz * v - oy * w - Ro * w;
+            double lx = oz * v - oy * w - Rx * w;
             double ly = ox * w - oz * u;
-            double lz = oy * u - ox * v + Ro * (u + meanU);
+            double lz = oy * u - ox * v + Rz * (u + meanU);
             ph[0][i] = lx; ph[1][i] = ly; ph[2][i] = lz;
         }
     }
@@ -799,13 +808,15 @@ static void linear_o_pencil(cx* rhs_o, const cx* nl_o, const cx* nl_op, const cx
                             const double* U, const double* dU, const double* T0, const double* T2, double Re,
                             double dt, double kmn, double alpha, double beta, cx* tmp) {
     const size_t n1 = (size_t)(N + 1);
+    const double sq = plant_sign(4), lu = plant_sign(3);
     for (int i = 2; i <= N; i++) {
         tmp[i] = cmk(0.0, 0.0);
         for (int j = 0; j <= N; j++) {
             size_t inc1 = n1 * (i - 1) + j;
             tmp[i] = cadd(tmp[i], cmul(rhs_o[j], cmk(T2[inc1] * 0.5 / Re * dt + (1 - kmn * dt * 0.5 / Re) * T0[inc1],
-                                                     -alpha * dt * 0.5 * U[i - 1] * T0[inc1])));
-            tmp[i] = cadd(tmp[i], cmul(cadd(rhs_v_p[j], rhs_v[j]), cmk(0.0, -0.5 * beta * dU[i - 1] * dt * T0[inc1])));
+                                                     -(sq * alpha) * dt * 0.5 * U[i - 1] * T0[inc1])));
+            tmp[i] = cadd(tmp[i], cmul(cadd(rhs_v_p[j], rhs_v[j]),
+                                       cmk(0.0, -0.5 * (lu * beta) * dU[i - 1] * dt * T0[inc1])));
         }
     }
     for (int i = 2; i <= N; i++)
@@ -1194,7 +1205,7 @@ static double seed_velocity_bound(const rpcf_params* prm, const double* v, const
 }
 
 int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int kmax, int degmax, double kexp,
-                    double* v_hat, double* eta_hat, long stride) {
+                    int peak_bound, double* v_hat, double* eta_hat, long stride) {
     int nx, ny, nz, pz, hnx;
     if (check_sizes(prm, &nx, &ny, &nz, &pz, &hnx) != RPCF_OK) return RPCF_EINVAL;
     if (!v_hat || !eta_hat || stride < nz || degmax < 0 || degmax + 4 >= nz || kmax < 0) return RPCF_EINVAL;
@@ -1238,10 +1249,11 @@ int rpcf_seed_state(const rpcf_params* prm, uint64_t seed, double amplitude, int
             }
         }
     }
-    /* amplitude: the bound of the peak perturbation velocity */
-    const double bound = seed_velocity_bound(prm, v_hat, eta_hat, stride, nz, ny, hnx);
+    /* amplitude: the factor of the Gaussian coefficients, or (peak_bound) the bound of the
+     * peak perturbation velocity */
+    const double bound = peak_bound ? seed_velocity_bound(prm, v_hat, eta_hat, stride, nz, ny, hnx) : 1.0;
     if (bound > 0.0) {
-        const double f = amplitude / bound;
+        const double f = peak_bound ? amplitude / bound : amplitude;
         for (size_t p = 0; p < P; p++)
             for (int k = 0; k < 2 * nz; k++) {
                 v_hat[2 * (size_t)stride * p + k] *= f;

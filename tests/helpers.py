@@ -31,16 +31,19 @@ def params(name, **over):
     return rpcf.make_params(**base)
 
 
-# seeded initial-field recipes (amplitude = bound of the peak perturbation velocity, kmax,
-# degmax, kexp).  C0, C1, C2 are the benchmark configurations; the test presets TINY and
-# REFCOUETTE use a larger perturbation so that their nonlinear term stands well above the
-# round-off of the Coriolis term of the base flow (Ro U), against which it is compared.
+# seeded initial-field recipes (amplitude, kmax, degmax, kexp, peak_bound).  peak_bound False:
+# the Gaussian coefficients times the amplitude, as the benchmark configurations state the
+# recipe (C0); True: the field scaled so that the amplitude bounds its peak velocity -- for
+# the channel and rotating Couette configurations, whose literal fields blow up within the
+# first steps at their time step (C1, C2), and for the test presets TINY and REFCOUETTE,
+# whose nonlinear term must stand well above the round-off of the Coriolis term of the base
+# flow (Ro U), against which it is compared.
 IC = {
-    "C0": (1e-2, 4, 8, 0.0),
-    "C1": (0.1, 16, 24, 5.0 / 6.0),
-    "C2": (0.05, 16, 24, 5.0 / 6.0),
-    "TINY": (0.2, 4, 8, 0.0),
-    "REFCOUETTE": (0.2, 4, 8, 0.0),
+    "C0": (1e-2, 4, 8, 0.0, False),
+    "C1": (0.1, 16, 24, 5.0 / 6.0, True),
+    "C2": (0.05, 16, 24, 5.0 / 6.0, True),
+    "TINY": (0.2, 4, 8, 0.0, True),
+    "REFCOUETTE": (0.2, 4, 8, 0.0, True),
 }
 
 

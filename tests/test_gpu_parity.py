@@ -107,7 +107,7 @@ def test_velocity_and_nonlinear_match(name):
     g.close(), o.close()
 
 
-@pytest.mark.parametrize("name", ["C0", "REFCOUETTE", "TINY"])
+@pytest.mark.parametrize("name", ["REFCOUETTE", "TINY"])  # C0: tests/test_configs.py
 def test_one_and_hundred_steps(name):
     prm = params(name)
     g, o = pair(prm)
@@ -122,24 +122,6 @@ def test_one_and_hundred_steps(name):
     a, b = g.get_state(), o.get_state()
     print(name, "100 steps", rel(a[0], b[0]), rel(a[1], b[1]))
     assert rel(a[0], b[0]) < 1e-9 and rel(a[1], b[1]) < 1e-9
-    g.close(), o.close()
-
-
-def test_channel_c1_one_step():
-    """the headline grid 128x129x128 channel with constant flow rate"""
-    prm = params("C1")
-    g, o = pair(prm)
-    v0, e0 = rpcf.seed_state(o.dll, prm, 0, *IC["C1"])
-    for sim in (g, o):
-        sim.set_state(v0, e0)
-        sim.bootstrap()
-    a, b = g.get_state(), o.get_state()
-    print("C1 1 step", rel(a[0], b[0]), rel(a[1], b[1]))
-    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
-    g.step(2), o.step(2)
-    a, b = g.get_state(), o.get_state()
-    print("C1 3 steps", rel(a[0], b[0]), rel(a[1], b[1]))
-    assert rel(a[0], b[0]) < 1e-11 and rel(a[1], b[1]) < 1e-11
     g.close(), o.close()
 
 

@@ -18,7 +18,7 @@ def write_para(path, prefix, start_type=0, start=0, end=6, save=3, rsave=4, ext=
     lines = ["24 24 64", "2.0 1.0", "500 0.2 0.01", f"{start_type} {start} {end} {save} {rsave}", prefix + "_out",
              prefix + "_rec"]
     if ext:
-        lines += ["0.0 1.0 0.0 0 0", "3 0.05 4 8 0.0"]
+        lines += ["0.0 1.0 0.0 0 0", "3 0.05 4 8 0.0 1"]
     open(path, "w").write("\n".join(lines) + "\n")
 
 

@@ -7,7 +7,7 @@ Reference tests restated here (their assertions, not their code):
   tests/test_transform.cpp:13-151, 218-255   padded round trip / padded->unpadded
   tests/test_nonlinear.cpp:103-143, 527-603  N_v, N_eta closed forms (Ro = 0)
   tests/test_getUVW.cpp:61-205               velocity/vorticity recovery
-The reference's tolerance is isEqual(a, b, 1e-8) (source/operation.cpp:135-148).
+The reference's tolerance is isEqual(a, b, 1e-8) (source/operation.cpp:98-111).
 """
 import ctypes
 
