@@ -22,9 +22,14 @@
 
 #include "../../include/rpcf.h"
 #include "../../include/rpcf_group.h"
+#include "../../include/rpcf_static.h"
 #include "host_prep.hpp"
 #include "k_linear.hpp"
 #include "k_transform.hpp"
+
+// Static device scratch of the entry points that allocate nothing (include/rpcf_static.h):
+// each GPU holds its own copy, RPCF_STATIC_SCRATCH_BYTES, until the process ends.
+static __device__ __align__(256) unsigned char scratch[RPCF_STATIC_SCRATCH_BYTES];
 
 namespace {
 
@@ -654,6 +659,27 @@ int autotune_tiles(rpcf_ctx* c) {
     return RPCF_OK;
 }
 
+// Device scratch of the entry points that allocate nothing (rpcf_spec_to_phys_static).  In
+// host code `scratch` is only a host-side stand-in for the symbol: its device address, one
+// per GPU, is looked up once per device while that device is current (init_ctx) and kept
+// in g_scratch_dev; kernels and copies take that address, never the name.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_setup(rpcf_ctx* c) {
+    if (c->device < 0 || c->device >= kMaxDevices)
+        return fail(c, RPCF_EDEVICE, "device %d outside 0..%d", c->device, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[c->device]) return RPCF_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(c, RPCF_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[c->device] = p;
+    return RPCF_OK;
+}
+
 void free_ctx(rpcf_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -684,6 +710,7 @@ int init_ctx(rpcf_ctx* c) {
     int rc;
     c->fac = 1.0 / d.mx / d.my;  // normalize factor, source/transform.cu:267
     HIPCHK(c, hipGetDevice(&c->device));
+    RCCHK(scratch_setup(c));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     int lim = 0;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lim > 0)
@@ -1259,6 +1286,15 @@ int rpcf_bootstrap(rpcf_ctx* c) {
 }  // extern "C"
 
 namespace {
+// Pencils of a caller's spectral array (`stride` complex numbers apart) packed to nz apart.
+__global__ __launch_bounds__(256) void k_gather_pencils(const double2* __restrict__ src, long stride,
+                                                        double2* __restrict__ dst, int nz, long total) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long p = i / nz;
+        dst[i] = src[p * stride + (i - p * nz)];
+    }
+}
+
 // The backward transform of one field on the context's stream: Chebyshev s2p (pencils f,
 // nz contiguous) -> planes y -> span C2C -> planes w -> x C2R -> ph; y and w are the
 // caller's intermediates of hnx*planes*ny and planes*my*nxh complex values.
@@ -1339,6 +1375,42 @@ int rpcf_phys_to_spec(rpcf_ctx* c, const double* phys, double* spec, long stride
     if (!rc) rc = download_spec(c, spec, f, stride);
     for (void* p : {(void*)f, (void*)y, (void*)z, (void*)ph})
         if (p) (void)hipFree(p);
+    return rc;
+}
+
+int rpcf_spec_to_phys_static(rpcf_ctx* c, const double* spec_dev, double* phys_dev, long stride, int padded) {
+    if (!c) return RPCF_EINVAL;
+    if (!spec_dev || !phys_dev) return fail(c, RPCF_EINVAL, "rpcf_spec_to_phys_static: null array");
+    if (((uintptr_t)spec_dev & 15) || ((uintptr_t)phys_dev & 7))
+        return fail(c, RPCF_EINVAL, "rpcf_spec_to_phys_static: misaligned device array");
+    const Dims d = c->d;
+    if (stride < d.nz) return fail(c, RPCF_EINVAL, "rpcf_spec_to_phys_static: pencil stride %ld < nz = %d", stride, d.nz);
+    const int planes = padded ? d.pz : d.nz;
+    auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bf = up256(sizeof(double2) * spec_elems(c));
+    const size_t by = up256(sizeof(double2) * (size_t)d.hnx * planes * d.ny);
+    const size_t bw = up256(sizeof(double2) * (size_t)planes * d.my * d.nxh);
+    if (bf + by + bw > RPCF_STATIC_SCRATCH_BYTES)
+        return fail(c, RPCF_EINVAL, "rpcf_spec_to_phys_static: needs %zu bytes of scratch, the maximum is %zu",
+                    bf + by + bw, (size_t)RPCF_STATIC_SCRATCH_BYTES);
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = (unsigned char*)g_scratch_dev[c->device];
+    }
+    if (!base) return fail(c, RPCF_ESTATE, "rpcf_spec_to_phys_static: no scratch on device %d", c->device);
+    double2* f = (double2*)base;
+    double2* y = (double2*)(base + bf);
+    double2* w = (double2*)(base + bf + by);
+    int rc = sync_ctx(c);
+    if (rc) return rc;
+    const long total = (long)spec_elems(c);
+    const int blocks = (int)std::min<long>((total + 255) / 256, 4L * c->ncu);
+    k_gather_pencils<<<blocks, 256, 0, c->stream>>>((const double2*)spec_dev, stride, f, d.nz, total);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, RPCF_EDEVICE, "rpcf_spec_to_phys_static launch: %s", hipGetErrorString(e));
+    rc = enq_spec_to_phys(c, f, y, w, phys_dev, padded);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, RPCF_EDEVICE, "spec_to_phys_static sync");
     return rc;
 }
 

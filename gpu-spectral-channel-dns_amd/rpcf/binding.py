@@ -113,6 +113,8 @@ def load(path):
         "rpcf_group_step": (ctypes.c_int, [_vp, ip, ctypes.c_int, ctypes.c_int]),
         "rpcf_group_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
         "rpcf_group_cache_hits": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long)]),
+        # include/rpcf_static.h: device arrays, no allocation
+        "rpcf_spec_to_phys_static": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, ctypes.c_int]),
     }
     for name, (res, args) in list(sig.items()) + list(group_sig.items()):
         fn = getattr(dll, name, None)

@@ -14,6 +14,7 @@ from helpers import IC, params
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
 HEADER = os.path.join(INCLUDE, "rpcf.h")
 GROUP_HEADER = os.path.join(INCLUDE, "rpcf_group.h")
+STATIC_HEADER = os.path.join(INCLUDE, "rpcf_static.h")
 
 
 def declared(header=HEADER):
@@ -43,6 +44,16 @@ def test_hip_library_exports_the_group_api():
     dll = ctypes.CDLL(rpcf.HIP_LIB)
     names = declared(GROUP_HEADER)
     assert "rpcf_group_step" in names and "rpcf_member_init" in names
+    missing = [n for n in names if not hasattr(dll, n)]
+    assert not missing, missing
+
+
+def test_hip_library_exports_the_static_api():
+    """include/rpcf_static.h (entry points that allocate nothing) is implemented by the HIP
+    library"""
+    dll = ctypes.CDLL(rpcf.HIP_LIB)
+    names = declared(STATIC_HEADER)
+    assert names == ["rpcf_spec_to_phys_static"]
     missing = [n for n in names if not hasattr(dll, n)]
     assert not missing, missing
 
