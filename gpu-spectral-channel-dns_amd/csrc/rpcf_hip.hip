@@ -1632,6 +1632,47 @@ int rpcf_group_step(rpcf_ctx* c, const int* slots, int n, int nsteps) {
     return group_steps(c, ids, nsteps);
 }
 
+int rpcf_member_adopt(rpcf_ctx* c, int slot, rpcf_ctx* src, int src_slot) {
+    if (!c || !src) return RPCF_EINVAL;
+    if (c == src) return fail(c, RPCF_EINVAL, "rpcf_member_adopt: the source is the group itself");
+    if (slot < 0 || slot >= c->capacity) return fail(c, RPCF_EINVAL, "slot %d outside 0..%d", slot, c->capacity - 1);
+    if (src_slot < 0 || src_slot >= src->capacity)
+        return fail(c, RPCF_EINVAL, "source slot %d outside 0..%d", src_slot, src->capacity - 1);
+    if (!same_shape(&src->shape, &c->shape) || src->shape.precision != c->shape.precision)
+        return fail(c, RPCF_EINVAL, "rpcf_member_adopt: the source group has another grid shape or precision");
+    const SlotHost sh = src->slots[src_slot];
+    if (!sh.configured || !sh.have_state)
+        return fail(c, RPCF_ESTATE, "rpcf_member_adopt: source slot %d holds no member with a state", src_slot);
+    int rc = sync_ctx(src);
+    if (rc) return fail(c, rc, "rpcf_member_adopt: source: %s", src->err);
+    if ((rc = sync_ctx(c))) return rc;
+    hipError_t e = hipSuccess;
+    auto cp = [&](auto* dst, const auto* from, long stride) {
+        if (e == hipSuccess && stride > 0)
+            e = hipMemcpyAsync(dst + slot * stride, from + src_slot * stride, sizeof(*dst) * (size_t)stride,
+                               hipMemcpyDeviceToDevice, c->stream);
+    };
+    cp(c->F, src->F, c->sF);
+    cp(c->SV, src->SV, c->sS);
+    cp(c->SE, src->SE, c->sS);
+    cp(c->NL, src->NL, c->sNL);
+    cp(c->MINV, src->MINV, c->sM);
+    cp(c->MINVH, src->MINVH, c->sMH);
+    cp(c->TF, src->TF, c->sTf);
+    cp(c->T4Q, src->T4Q, c->sQ);
+    cp(c->T2Q, src->T2Q, c->sQ);
+    cp(c->U, src->U, c->sU);
+    cp(c->DU, src->DU, c->sU);
+    cp(c->DDU, src->DDU, c->sU);
+    cp(c->MEANU, src->MEANU, c->sPz);
+    cp(c->MP, src->MP, 1L);
+    cp(c->PH, src->PH, 1L);
+    if (e != hipSuccess) return fail(c, RPCF_EDEVICE, "rpcf_member_adopt copy: %s", hipGetErrorString(e));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->slots[slot] = sh;
+    return RPCF_OK;
+}
+
 int rpcf_group_graphs(rpcf_ctx* c, int enable) {
     if (!c) return RPCF_EINVAL;
     int rc = sync_ctx(c);

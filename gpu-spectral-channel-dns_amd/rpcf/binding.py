@@ -113,6 +113,7 @@ def load(path):
         "rpcf_group_step": (ctypes.c_int, [_vp, ip, ctypes.c_int, ctypes.c_int]),
         "rpcf_group_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
         "rpcf_group_cache_hits": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long)]),
+        "rpcf_member_adopt": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int]),
         # include/rpcf_static.h: device arrays, no allocation
         "rpcf_spec_to_phys_static": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, ctypes.c_int]),
     }
@@ -344,6 +345,11 @@ class Group:
 
     def graphs(self, enable=True):
         self._check(self.dll.rpcf_group_graphs(self.h, int(bool(enable))), "group_graphs")
+
+    def adopt(self, slot, src, src_slot=0):
+        """move the member of `src` (another group of this shape and precision) into `slot`:
+        device-to-device copies of its operators, inverses, state and history"""
+        self._check(self.dll.rpcf_member_adopt(self.h, int(slot), src.h, int(src_slot)), "member_adopt")
 
     def cache_hits(self):
         """member initialisations whose inverses were copied from another slot"""

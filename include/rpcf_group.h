@@ -51,6 +51,14 @@ int rpcf_member_get_state(rpcf_ctx* group, int slot, double* v_hat, double* eta_
  * The velocity fields of the listed members are formed from their states. */
 int rpcf_group_bootstrap(rpcf_ctx* group, const int* slots, int n);
 int rpcf_group_step(rpcf_ctx* group, const int* slots, int n, int nsteps);
+/* Move the member in slot src_slot of `src` (another group of the same grid shape and
+ * precision) into `slot` of `group`: device-to-device copies of its operators, inverses,
+ * state, velocity fields and AB2 history on the group's stream; the source slot is left as
+ * it is.  A serving loop builds a newcomer (rpcf_member_init, rpcf_member_set_state,
+ * rpcf_group_bootstrap) in a one-slot staging group on that group's own stream while the
+ * running group steps, and adopts it between two steps, so the running members wait only
+ * for the copy.  Waits for both groups' streams. */
+int rpcf_member_adopt(rpcf_ctx* group, int slot, rpcf_ctx* src, int src_slot);
 /* enable = 1: rpcf_group_step replays one captured HIP graph of a grouped step per step
  * (captured again whenever the slot list changes; not while rpcf_timing is on). */
 int rpcf_group_graphs(rpcf_ctx* group, int enable);
