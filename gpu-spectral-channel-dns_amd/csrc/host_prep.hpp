@@ -123,4 +123,104 @@ inline void mul_one_minus_y2(const double* a, double* b, int n) {
     for (int k = 0; k < 2 * n; k++) b[k] = a[k] - yya[k];
 }
 
+// ---- tables of the split-FP16 Chebyshev products (mixed form, k_cheb_mfma.hpp) ---------
+// cos(pi n / m) with n reduced modulo 2m first
+inline long double cospi_ratio(long n, long m) {
+    const long double PI = 3.141592653589793238462643383279502884L;
+    n %= 2 * m;
+    return cosl(PI * (long double)n / (long double)m);
+}
+
+// power-of-two exponent e with max|T| 2^e in [2^14, 2^15)
+inline int table_scale(const std::vector<long double>& T) {
+    long double m = 0;
+    for (long double v : T) m = fabsl(v) > m ? fabsl(v) : m;
+    if (!(m > 0)) return 0;
+    int ex;
+    (void)frexpl(m, &ex);
+    return 15 - ex;
+}
+
+// Append the fragments of a rows x cols table (row-major) scaled by 2^scale, in the A-operand
+// lane order of v_mfma_f32_16x16x32_f16: fragment (row tile rt, k-step s) is
+// [hi, lo][lane l][i] = T[16 rt + l % 16][32 s + 8 (l / 16) + i] (zero outside the table), with
+// hi = fp16(x), lo = fp16(x - hi).
+inline void mfma_fragments(const std::vector<long double>& T, int rows, int cols, int scale,
+                           std::vector<_Float16>& out) {
+    const int RT = (rows + 15) / 16, KS = (cols + 31) / 32;
+    for (int rt = 0; rt < RT; rt++)
+        for (int s = 0; s < KS; s++)
+            for (int part = 0; part < 2; part++)
+                for (int l = 0; l < 64; l++)
+                    for (int i = 0; i < 8; i++) {
+                        const int r = 16 * rt + l % 16, k = 32 * s + 8 * (l / 16) + i;
+                        const double x = r < rows && k < cols ? (double)ldexpl(T[(size_t)r * cols + k], scale) : 0.0;
+                        const _Float16 hi = (_Float16)(float)x;
+                        out.push_back(part == 0 ? hi : (_Float16)(float)(x - (double)(float)hi));
+                    }
+}
+
+// s2p of length 2N: E[q][j] = cos(pi 2j q/(2N)) (q <= N, j <= N/2) and
+// O[q][j] = cos(pi (2j+1) q/(2N)) (q < N, j < N/2), scaled by 2^14.
+inline void s2p_tables(int N, std::vector<_Float16>& out) {
+    const int ne = N / 2 + 1, no = N / 2;
+    std::vector<long double> E((size_t)(N + 1) * ne), O((size_t)N * no);
+    for (int q = 0; q <= N; q++)
+        for (int j = 0; j < ne; j++) E[(size_t)q * ne + j] = cospi_ratio((long)2 * j * q, 2L * N);
+    for (int q = 0; q < N; q++)
+        for (int j = 0; j < no; j++) O[(size_t)q * no + j] = cospi_ratio((long)(2 * j + 1) * q, 2L * N);
+    mfma_fragments(E, N + 1, ne, 14, out);
+    mfma_fragments(O, N, no, 14, out);
+}
+
+// p2s + derivative + unpadded s2p of length 2N (see k_p2s_mfma): M0 = S P, M1 = S D P with
+// P the padded p2s (source/transform.cu:513-536, 538-658), D the Chebyshev derivative
+// (ddz_sm, source/RPCFKernels.cu:476-520), S the values at the nz collocation points;
+// then the folded tables Q0s, Q0a, Q1s, Q1a.  s0, s1: scales of the M0 and M1 tables.
+inline void p2s_tables(int N, std::vector<_Float16>& out, int* s0, int* s1) {
+    const int nz = N + 1, pz = 2 * N + 1;
+    std::vector<long double> P((size_t)nz * pz), DP((size_t)nz * pz, 0.0L), M0((size_t)nz * pz), M1((size_t)nz * pz);
+    for (int k = 0; k < nz; k++)
+        for (int q = 0; q < pz; q++) {
+            const long double w = (q == 0 || q == 2 * N) ? 1.0L : 2.0L;
+            P[(size_t)k * pz + q] = w * cospi_ratio((long)k * q, 2L * N) / (2.0L * N) * (k == 0 ? 0.5L : 1.0L);
+        }
+    for (int k = 0; k < nz; k++)
+        for (int j = k + 1; j < nz; j += 2)
+            for (int q = 0; q < pz; q++) DP[(size_t)k * pz + q] += 2.0L * j * P[(size_t)j * pz + q] * (k == 0 ? 0.5L : 1.0L);
+    for (int p = 0; p < nz; p++)
+        for (int q = 0; q < pz; q++) {
+            long double a = 0, b = 0;
+            for (int k = 0; k < nz; k++) {
+                const long double c = cospi_ratio((long)k * p, (long)N);
+                a += c * P[(size_t)k * pz + q];
+                b += c * DP[(size_t)k * pz + q];
+            }
+            M0[(size_t)p * pz + q] = a;
+            M1[(size_t)p * pz + q] = b;
+        }
+    const int rs = N / 2 + 1, ra = N / 2;
+    std::vector<long double> q0s((size_t)rs * (N + 1)), q0a((size_t)ra * N), q1s((size_t)rs * N), q1a((size_t)ra * (N + 1));
+    for (int p = 0; p < rs; p++)
+        for (int q = 0; q <= N; q++) {
+            const long double m0 = M0[(size_t)p * pz + q], m0r = M0[(size_t)p * pz + 2 * N - q];
+            const long double m1 = M1[(size_t)p * pz + q], m1r = M1[(size_t)p * pz + 2 * N - q];
+            q0s[(size_t)p * (N + 1) + q] = q < N ? (m0 + m0r) / 2 : m0;
+            if (q < N) q1s[(size_t)p * N + q] = (m1 - m1r) / 2;
+            if (p < ra) {
+                if (q < N) q0a[(size_t)p * N + q] = (m0 - m0r) / 2;
+                q1a[(size_t)p * (N + 1) + q] = q < N ? (m1 + m1r) / 2 : m1;
+            }
+        }
+    std::vector<long double> all0(q0s), all1(q1s);
+    all0.insert(all0.end(), q0a.begin(), q0a.end());
+    all1.insert(all1.end(), q1a.begin(), q1a.end());
+    *s0 = table_scale(all0);
+    *s1 = table_scale(all1);
+    mfma_fragments(q0s, rs, N + 1, *s0, out);
+    mfma_fragments(q0a, ra, N, *s0, out);
+    mfma_fragments(q1s, rs, N, *s1, out);
+    mfma_fragments(q1a, ra, N + 1, *s1, out);
+}
+
 }  // namespace rpcf_host

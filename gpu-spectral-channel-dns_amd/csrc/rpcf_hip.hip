@@ -26,6 +26,7 @@
 #include "host_prep.hpp"
 #include "k_linear.hpp"
 #include "k_transform.hpp"
+#include "k_cheb_mfma.hpp"
 
 // Static device scratch of the entry points that allocate nothing (include/rpcf_static.h):
 // each GPU holds its own copy, RPCF_STATIC_SCRATCH_BYTES, until the process ends.
@@ -82,6 +83,11 @@ struct rpcf_ctx {
     float* twf = nullptr;
     int cap_x = 0, cap_y = 0, cap_zp = 0, cap_zn = 0;  // fft_capacity of each plan
     int tv_rt = 0, tv_s = 0;                           // MFMA row tiles and k-steps of the operator products
+    // mixed form: Chebyshev transforms as split-FP16 products (k_cheb_mfma.hpp)
+    int use_mfma = 0;
+    _Float16* MT = nullptr;  // s2p fragments, then p2s fragments
+    long mt_p2s = 0;         // offset (halves) of the p2s fragments
+    int mt_s0 = 0, mt_s1 = 0;
     // per-slot arrays ([capacity][...]) and their slot strides in elements
     double2* F = nullptr;     // [6][P][nz] u, v, w, ox, oy, oz
     double2* SV = nullptr;    // [P][nz] v-hat (u0 in pencil 0)
@@ -409,6 +415,22 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
         rc = launch(
             c, KID_S2P,
             [&] {
+                if constexpr (!f64) {
+                    if (c->use_mfma) {
+                        dim3 gm((d.ny + cheb_mfma::kPencils - 1) / cheb_mfma::kPencils, d.hnx, 6 * n);
+                        switch (2 * d.N) {
+#define RPCF_CASE(L)                                                                                               \
+    case L:                                                                                                        \
+        if constexpr (L <= 384)                                                                                    \
+            k_s2p_mfma<(L <= 384 ? L : 384)><<<gm, 256, 0, c->stream>>>(c->F, fS, c->sF, Y, yS, c->sY, d,          \
+                                                                       (const h8_t*)c->MT, ids);                  \
+        break;
+                            RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                        }
+                        return;
+                    }
+                }
                 switch (2 * d.N) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
@@ -492,6 +514,26 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
         if (rc) return rc;
     }
     // the AB2 history swap ran inside k_span_fwd: the new N lands in the old "previous" buffers
+    if constexpr (!f64) {
+        if (c->use_mfma) {
+            dim3 g((d.ny + cheb_mfma::kPencils - 1) / cheb_mfma::kPencils, d.hnx, n);
+            return launch(
+                c, KID_P2S_NL,
+                [&] {
+                    switch (2 * d.N) {
+#define RPCF_CASE(L)                                                                                                 \
+    case L:                                                                                                          \
+        if constexpr (L <= 384)                                                                                      \
+            k_p2s_mfma<(L <= 384 ? L : 384)><<<g, 64 * cheb_mfma::Shape<(L <= 384 ? L : 384)>::RTS, 0, c->stream>>>( \
+                Y, yS, c->sY, c->NL, c->sNL, d, (const h8_t*)(c->MT + c->mt_p2s), c->mt_s0, c->mt_s1, mb);          \
+        break;
+                        RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                    }
+                },
+                cap);
+        }
+    }
     if (c->TB_nl2) {
         const int TB = c->TB_nl2, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
@@ -604,7 +646,7 @@ int autotune_tiles(rpcf_ctx* c) {
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     // k_velocity_tile and k_p2s_nl2 run for compiled Chebyshev lengths only
-    const bool tunable[6] = {true, true, true, true, c->TB_nl2 > 0, compiled};
+    const bool tunable[6] = {!c->use_mfma, true, true, true, c->TB_nl2 > 0 && !c->use_mfma, compiled};
     if (mode == "half") {
         for (int i = 0; i < 6; i++)
             if (tunable[i] && *tile_ref(c, i) >= 2) *tile_ref(c, i) /= 2;
@@ -686,7 +728,7 @@ void free_ctx(rpcf_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     void* ptrs[] = {c->F,   c->SV, c->SE, c->NL,  c->Y,     c->W,  c->Z2, c->WL, c->MINV, c->TF,  c->T4Q,
                     c->T2Q, c->U,  c->DU, c->DDU, c->MEANU, c->MP, c->PH, c->T0, c->T2,   c->T4,  c->T0t,
-                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->PAIRS};
+                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->PAIRS, c->MT};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -701,6 +743,26 @@ template <typename K>
 void allow_lds(K kernel, int bytes) {
     if (hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
         (void)hipGetLastError();
+}
+
+// Host fragments of the split-FP16 Chebyshev products of one length 2N, built once per process.
+struct MfmaTables {
+    std::vector<_Float16> h;  // s2p fragments, then p2s fragments
+    long p2s_offset = 0;
+    int s0 = 0, s1 = 0;
+};
+std::mutex g_mt_mu;
+std::map<int, MfmaTables> g_mt;
+
+const MfmaTables& mfma_tables(int N) {
+    std::lock_guard<std::mutex> lk(g_mt_mu);
+    auto it = g_mt.find(N);
+    if (it != g_mt.end()) return it->second;
+    MfmaTables t;
+    rpcf_host::s2p_tables(N, t.h);
+    t.p2s_offset = (long)t.h.size();
+    rpcf_host::p2s_tables(N, t.h, &t.s0, &t.s1);
+    return g_mt.emplace(N, std::move(t)).first->second;
 }
 
 // allocation and shape set-up of a context of `capacity` slots; on failure the caller frees it
@@ -811,6 +873,20 @@ int init_ctx(rpcf_ctx* c) {
     if ((rc = make_plan(c, 4 * d.N, tw, &c->pl_zp, &c->cap_zp))) return rc;
     tw += 2 * 4 * d.N;
     if ((rc = make_plan(c, 2 * d.N, tw, &c->pl_zn, &c->cap_zn))) return rc;
+    // mixed form: fragments of the split-FP16 Chebyshev products (k_cheb_mfma.hpp) for the
+    // compiled lengths up to 2N = 384; RPCF_MIXED_MFMA=0 keeps the FFT-based kernels
+    if (c->mixed && 2 * d.N <= 384) {
+        const char* e = getenv("RPCF_MIXED_MFMA");
+        if (!(e && e[0] == '0')) {
+            const MfmaTables& t = mfma_tables(d.N);
+            if ((rc = dalloc(c, &c->MT, t.h.size()))) return rc;
+            RCCHK(copy_sync(c, c->MT, t.h.data(), sizeof(_Float16) * t.h.size(), hipMemcpyHostToDevice));
+            c->mt_p2s = t.p2s_offset;
+            c->mt_s0 = t.s0;
+            c->mt_s1 = t.s1;
+            c->use_mfma = 1;
+        }
+    }
 
     // LDS tiles, sized for three resident blocks per CU
     lim = c->lds_limit;

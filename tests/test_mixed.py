@@ -1,9 +1,16 @@
-"""The mixed-precision form (precision = 1): the transform intermediates and the x / span
-FFTs and the Lamb vector in FP32, the Chebyshev transforms, the nonlinear combination,
-the operators and the solves in FP64.  Bar: after 100 steps (200 for the rotating Couette
-case) every field stays within 1e-6 relative (max |a-b| / max |b|) of the FP64 run from
-the same initial field, and differs from it (the FP32 path ran)."""
+"""The mixed-precision form (precision = 1): the transform intermediates, the x / span
+FFTs and the Lamb vector in FP32; the wall-normal Chebyshev transforms as split-FP16 products
+on the matrix cores with FP32 accumulation (the padded s2p of the six fields; the padded p2s,
+derivative, nonlinear combination and unpadded s2p folded into three products); velocity
+recovery, the operators, the solves and the mean flow in FP64.  Bars: after 100 steps (200
+for the rotating Couette case) every field stays within 1e-6 relative (max |a-b| / max |b|)
+of the FP64 run from the same initial field, and differs from it (the FP32 path ran); the
+nonlinear term of one evaluation within 2e-6 (N_eta) and 2e-4 (N_v) of the CPU oracle: FP32
+round-off of the Lamb products, which the wall-normal derivative in N_v amplifies up to ~N^2
+for the high Chebyshev modes (the FFT-based FP32 form measures 6.5e-5 there at
+128x129x128; those modes are the ones the implicit fourth-order solve damps most)."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -53,6 +60,57 @@ def test_mixed_within_1e6_of_fp64(name):
     print(name, "max relative error per field (v, eta, u, v, w, ox, oy, oz):", ["%.2e" % x for x in errs])
     assert max(errs) <= 1e-6
     assert max(errs) > 1e-13
+
+
+NL_CASES = {
+    # 2N = 64 (N = 32), Ro != 0, Couette
+    "TINY": (params("TINY"), IC["TINY"]),
+    # 2N = 256 (N = 128), channel, flow control
+    "C1": (params("C1"), IC["C1"]),
+    # 2N = 96 (N = 48): a length whose halves are not powers of two
+    "N48": (rpcf.make_params(mx=48, my=48, mz=192, n_pi_x=2.0, n_pi_y=1.0, Re=1500.0, Ro=0.3, dt=0.002,
+                             u_profile=(1.0, 0.0, -1.0), flow_control=1), (0.05, 6, 10, 0.0, True)),
+    # 2N = 24 (N = 12): the shortest compiled length, tiles of 7 rows
+    "N12": (rpcf.make_params(mx=24, my=24, mz=48, n_pi_x=2.0, n_pi_y=1.0, Re=800.0, Ro=0.1, dt=0.002,
+                             u_profile=(0.0, 1.0, 0.0)), (0.05, 4, 6, 0.0, True)),
+    # 2N = 384 (N = 192): the longest compiled Chebyshev length (seven row tiles per block)
+    "DEEP": (CASES["DEEP"][0], (0.05, 8, 8, 0.0, True)),
+}
+
+
+def nonlinear_of(lib, prm, ic, mfma=True):
+    old = os.environ.get("RPCF_MIXED_MFMA")
+    os.environ["RPCF_MIXED_MFMA"] = "1" if mfma else "0"
+    try:
+        sim = rpcf.Simulation(lib, prm)
+    finally:
+        if old is None:
+            del os.environ["RPCF_MIXED_MFMA"]
+        else:
+            os.environ["RPCF_MIXED_MFMA"] = old
+    with sim:
+        sim.set_state(*rpcf.seed_state(sim.dll, prm, 3, *ic))
+        sim.nonlinear()
+        return sim.get_nonlinear()
+
+
+@pytest.mark.parametrize("name", list(NL_CASES))
+def test_mixed_nonlinear_term_against_oracle(name):
+    """one nonlinear evaluation (six backward transforms, Lamb + Coriolis, three forward
+    transforms, rhsNonlinear, unpadded s2p) of the mixed form on the matrix cores against the
+    FP64 oracle, and against the mixed form with the FFT-based Chebyshev kernels"""
+    prm, ic = NL_CASES[name]
+    prm = fresh(prm)
+    ov, oe = nonlinear_of(rpcf.ORACLE_LIB, prm, ic)
+    prm.precision = 1
+    mv, me = nonlinear_of(rpcf.HIP_LIB, prm, ic, mfma=True)
+    fv, fe = nonlinear_of(rpcf.HIP_LIB, prm, ic, mfma=False)
+    errs = {"N_v": rel(mv, ov), "N_eta": rel(me, oe), "N_v (FFT form)": rel(fv, ov), "N_eta (FFT form)": rel(fe, oe),
+            "N_v MFMA vs FFT form": rel(mv, fv), "N_eta MFMA vs FFT form": rel(me, fe)}
+    print(name, {k: "%.2e" % v for k, v in errs.items()})
+    assert errs["N_v"] < 2e-4 and errs["N_eta"] < 2e-6
+    assert errs["N_v MFMA vs FFT form"] < 2e-4 and errs["N_eta MFMA vs FFT form"] < 2e-6
+    assert not np.array_equal(mv, fv)  # the two Chebyshev paths both ran
 
 
 def test_group_precision_must_match():
