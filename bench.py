@@ -79,9 +79,21 @@ def read_traffic(kernel):
     return float(m.group(1)) if m else None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(prm, v0, e0, steps, ref_state):
     """the FP64 oracle (oracle/rpcf_oracle.c) on this host's cores: create (matrix
-    build, not timed), bootstrap, then `steps` timed steps"""
+    build, not timed), bootstrap, then `steps` timed steps, each as its three parts
+    (nonlinear term, linear step = CN/AB2 right-hand sides + per-mode solves + mean modes,
+    velocity recovery) so that the linear part is reported on its own"""
     dll = rpcf.load(rpcf.ORACLE_LIB)
     dll.rpcf_oracle_threads.restype = ctypes.c_int
     threads = int(dll.rpcf_oracle_threads())
@@ -92,10 +104,13 @@ def cpu_baseline(prm, v0, e0, steps, ref_state):
         o.bootstrap()
         a = o.get_state()
         parity = max(float(np.abs(ref_state[i] - a[i]).max() / np.abs(a[i]).max()) for i in (0, 1))
-        t0 = time.perf_counter()
-        o.step(steps)
-        t_steps = time.perf_counter() - t0
-    return threads, t_init, t_steps / steps, parity
+        t_parts = [0.0, 0.0, 0.0]
+        for _ in range(steps):
+            for j, fn in enumerate((o.nonlinear, o.linear, o.velocity)):
+                t1 = time.perf_counter()
+                fn()
+                t_parts[j] += time.perf_counter() - t1
+    return threads, t_init, [t / steps for t in t_parts], parity
 
 
 def copy_probe(nbytes, reps):
@@ -112,11 +127,28 @@ def copy_probe(nbytes, reps):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
-def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64, precision=1):
+def kernel_table(dll, h):
+    """{kernel name: (summed ms, launches)} of the HIP-event timers of a context"""
+    dll.rpcf_timing_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_long)]
+    dll.rpcf_kernel_name.restype = ctypes.c_char_p
+    dll.rpcf_kernel_name.argtypes = [ctypes.c_int]
+    out = {}
+    for kid in range(dll.rpcf_num_kernels()):
+        tot, cnt = ctypes.c_double(), ctypes.c_long()
+        dll.rpcf_timing_get(h, kid, ctypes.byref(tot), ctypes.byref(cnt))
+        if cnt.value:
+            out[dll.rpcf_kernel_name(kid).decode()] = (tot.value, cnt.value)
+    return out
+
+
+def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64, precision=1, table_steps=10):
     """the same workload in another form -- precision 1, the mixed form (FP32 transform
-    intermediates, x/span FFTs and Lamb vector; FP64 Chebyshev transforms, operators and
-    parity-block solves), or 2 (FP64 with the parity-block solves): time per step and the
-    relative field error against the FP64 run after the same steps"""
+    intermediates, x/span FFTs and Lamb vector; Chebyshev transforms as split-FP16 products
+    on the matrix cores; FP64 velocity recovery, operators and parity-block solves), or 2
+    (FP64 with the parity-block solves): time per step, the relative field error against the
+    FP64 run after the same steps, and the per-kernel times of a separate pass with events
+    around every launch (run after the timed steps, so the error compares equal step counts)"""
     prm = rpcf.make_params(**C1)
     prm.precision = precision
     with rpcf.Simulation(rpcf.HIP_LIB, prm) as sim:
@@ -129,12 +161,20 @@ def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64, precision=1):
         device_sync()
         ms = (time.perf_counter() - t0) / steps * 1e3
         v, e = sim.get_state()
+        dll = sim.dll
+        dll.rpcf_timing_kernels.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        assert dll.rpcf_timing_kernels(sim.h, 0xFFFFFFFF) == 0
+        sim.step(table_steps)
+        table = {k: round(t / c, 5) for k, (t, c) in kernel_table(dll, sim.h).items()}
+        assert dll.rpcf_timing_kernels(sim.h, 0) == 0
     err = {"v": float(np.abs(v - final64[0]).max() / np.abs(final64[0]).max()),
            "eta": float(np.abs(e - final64[1]).max() / np.abs(final64[1]).max())}
     log(f"precision {precision}: {ms:.4f} ms/step ({ms64 / ms:.2f}x FP64), error after {1 + warmup + steps} steps {err}")
+    log(f"precision {precision} kernels (ms): {table}")
     return {"ms_per_step": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3), "steps_compared": 1 + warmup + steps,
-            "max_rel_error": err,
-            "dtype": "f32 transforms / f64 solves" if precision == 1 else "f64, parity-block solves"}
+            "max_rel_error": err, "kernel_avg_ms": table,
+            "dtype": "f32 transforms (split-f16 MFMA Chebyshev) / f64 solves" if precision == 1
+            else "f64, parity-block solves"}
 
 
 def ensemble_leg(n, nsteps):
@@ -145,10 +185,12 @@ def ensemble_leg(n, nsteps):
     ms = ensemble.mixed_members(n, seed=0, precision=1)
     wall, runs = ensemble.run_grouped(ms, nsteps)
     alone_total, alone_each, _ = ensemble.run_alone(ms, nsteps)
+    alone_g_total, alone_g_each, _ = ensemble.run_alone(ms, nsteps, graphs=True)
     per100_group = [r.seconds / nsteps * 100 for r in runs for _ in r.members]
     per100_alone = [t / nsteps * 100 for t in alone_each]
     gp = sum(ensemble.grid_points(m) for m in ms) * nsteps
-    log(f"ensemble {n} members x {nsteps} steps: grouped {wall:.3f}s, one at a time {alone_total:.3f}s")
+    log(f"ensemble {n} members x {nsteps} steps: grouped {wall:.3f}s, one at a time {alone_total:.3f}s "
+        f"(with graph replay {alone_g_total:.3f}s)")
     return {
         "members": n, "steps": nsteps, "dtype": "f32 transforms / f64 solves",
         "groups": {f"{r.shape[0]}x{r.shape[1]}x{r.shape[2]}": len(r.members) for r in runs},
@@ -158,6 +200,9 @@ def ensemble_leg(n, nsteps):
         "steps_per_s": round(n * nsteps / wall, 1),
         "steps_per_s_one_at_a_time": round(n * nsteps / alone_total, 1),
         "speedup": round(alone_total / wall, 2),
+        "one_at_a_time_graphs_s": round(alone_g_total, 4),
+        "steps_per_s_one_at_a_time_graphs": round(n * nsteps / alone_g_total, 1),
+        "speedup_vs_graphs_one_at_a_time": round(alone_g_total / wall, 2),
         "grid_point_steps_per_s": round(gp / wall, 1),
         "p50_s_per_100_steps": round(float(np.percentile(per100_group, 50)), 4),
         "p99_s_per_100_steps": round(float(np.percentile(per100_group, 99)), 4),
@@ -188,7 +233,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps for the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed oracle steps for the CPU baseline")
     ap.add_argument("--table-steps", type=int, default=10,
                     help="steps of the untimed pass that times every kernel (the per-kernel table)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -212,6 +257,10 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe, 3)
         return
+    # the CPU baseline's OpenMP threads stay close to one another; their count is the
+    # environment's OMP_NUM_THREADS, else this process's CPU share
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
 
     prm = rpcf.make_params(**C1)
     t0 = time.perf_counter()
@@ -285,12 +334,20 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu:
-        threads, t_init, t_cpu, parity = cpu_baseline(prm, v0, e0, args.cpu_steps, state1)
+        threads, t_init, parts, parity = cpu_baseline(prm, v0, e0, args.cpu_steps, state1)
+        t_cpu = sum(parts)
         cpu = {"value": round(t_cpu * 1e9 / npts, 2), "unit": "ns/gridpoint/step", "cores": threads,
-               "kind": "port", "sample": f"FP64 oracle (OpenMP, {threads} threads) on the same 128x129x128 "
-                                         f"channel: matrix build {t_init:.1f}s untimed, bootstrap, then "
-                                         f"{args.cpu_steps} timed steps ({t_cpu * 1e3:.0f} ms/step)"}
-        log(f"cpu baseline {t_cpu * 1e3:.1f} ms/step on {threads} threads; GPU-vs-oracle parity after 1 step {parity:.2e}")
+               "kind": "port",
+               "sample": f"FP64 oracle (OpenMP, {threads} threads, OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}) "
+                         f"on the same 128x129x128 channel: matrix build {t_init:.1f}s untimed, bootstrap, then "
+                         f"{args.cpu_steps} timed steps ({t_cpu * 1e3:.0f} ms/step)",
+               "ms_per_step": round(t_cpu * 1e3, 2),
+               "linear_ms_per_step": round(parts[1] * 1e3, 2),
+               "nonlinear_ms_per_step": round(parts[0] * 1e3, 2),
+               "velocity_ms_per_step": round(parts[2] * 1e3, 2),
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(), "cpu_share": len(os.sched_getaffinity(0))}
+        log(f"cpu baseline {t_cpu * 1e3:.1f} ms/step (linear {parts[1] * 1e3:.1f}) on {threads} threads "
+            f"of {cpu['cpu_model']}; GPU-vs-oracle parity after 1 step {parity:.2e}")
 
     out = {
         "metric": "ns/gridpoint/step at 128x129x128 (FP64)",

@@ -115,23 +115,38 @@ def run_grouped(members, nsteps, lib=B.HIP_LIB, graphs=True, keep_states=False):
     return wall, runs
 
 
-def run_alone(members, nsteps, lib=B.HIP_LIB, keep_states=False):
-    """The baseline: each member alone, one after the other (bootstrap untimed)."""
+def run_alone(members, nsteps, lib=B.HIP_LIB, keep_states=False, graphs=False):
+    """The baseline: each member alone, one after the other (bootstrap untimed).
+    graphs=False: plain single simulations (rpcf_step, one launch per kernel);
+    graphs=True: each member as a group of one replaying a captured HIP graph per step, so
+    that the comparison with run_grouped isolates the batching of members."""
     dll = B.load(lib)
     total = 0.0
     per = []
     states = {}
     for i, m in enumerate(members):
-        with B.Simulation(lib, m.prm) as sim:
-            sim.set_state(*initial_state(dll, m))
-            sim.bootstrap()
-            t0 = time.perf_counter()
-            sim.step(nsteps)
-            dt = time.perf_counter() - t0
-            total += dt
-            per.append(dt)
-            if keep_states:
-                states[i] = sim.get_state()
+        if graphs:
+            with B.Group(lib, m.prm, 1) as g:
+                g.init(0, m.prm)
+                g.set_state(0, *initial_state(dll, m))
+                g.graphs(True)
+                g.bootstrap([0])
+                t0 = time.perf_counter()
+                g.step([0], nsteps)
+                dt = time.perf_counter() - t0
+                if keep_states:
+                    states[i] = g.get_state(0)
+        else:
+            with B.Simulation(lib, m.prm) as sim:
+                sim.set_state(*initial_state(dll, m))
+                sim.bootstrap()
+                t0 = time.perf_counter()
+                sim.step(nsteps)
+                dt = time.perf_counter() - t0
+                if keep_states:
+                    states[i] = sim.get_state()
+        total += dt
+        per.append(dt)
     return total, per, states
 
 
