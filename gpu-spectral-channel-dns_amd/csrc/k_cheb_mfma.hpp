@@ -88,73 +88,83 @@ __device__ __forceinline__ void lds_max(unsigned* slot, float v) { atomicMax(slo
 }  // namespace cheb_mfma
 
 // -------------------------------------------------------------------------------------
-// Padded s2p of the six spectral fields (mixed form): F[f][P][nz] (FP64) ->
-// Y[f][kx][q][ky] (FP32), q = 0..2N.  f_q = sum_k a_k cos(pi k q / (2N)); with
-// E_q = sum_{k even} a_k cos(pi k q/(2N)) (q <= N) and O_q the odd sum (q < N):
-// f_q = E_q + O_q, f_{2N-q} = E_q - O_q, f_N = E_N.  Block: 16 pencils (ky0.., kx fixed) of
-// one field (blockIdx.z = member * 6 + field), 256 threads; wave w takes the row tiles
-// w, w + 4, .. of E and the same row tiles of O.  tab: the E fragments, then the O
-// fragments.
+// Padded s2p of the six spectral fields (mixed form): F[f][P][nz] (FP64, or FP32 when the
+// velocity recovery stores it so) -> Y[f][kx][q][ky] (FP32), q = 0..2N.
+// f_q = sum_k a_k cos(pi k q / (2N)); with E_q = sum_{k even} a_k cos(pi k q/(2N)) (q <= N)
+// and O_q the odd sum (q < N): f_q = E_q + O_q, f_{2N-q} = E_q - O_q, f_N = E_N.
+// Block: 16 pencils (ky0.., kx fixed) of one field (blockIdx.z = member * 6 + field), 256
+// threads.  Thread (b, m) = (tid / 16, tid % 16) loads the coefficients k = 16 m .. 16 m + 15
+// of pencil b (a contiguous 256- or 128-byte run), takes the pencil's largest magnitude
+// over its 16 lanes (shuffles, no LDS atomics) and writes the even and the odd eight of its
+// run, split and scaled, as 16-byte rows of the operand tiles (zeros past nz and for
+// missing pencils, so every tile entry a product reads is written).  Wave w then takes the
+// row tiles w, w + 4, .. of E and the same row tiles of O.  tab: the E fragments, then the
+// O fragments.
 // -------------------------------------------------------------------------------------
-template <int FN>
-__global__ __launch_bounds__(256) void k_s2p_mfma(const double2* __restrict__ F, long fstride, long sF,
+template <int FN, typename TI = double2>
+__global__ __launch_bounds__(256) void k_s2p_mfma(const TI* __restrict__ F, long fstride, long sF,
                                                   float2* __restrict__ Y, long ystride, long sY, Dims d,
                                                   const h8_t* __restrict__ tab, const int* __restrict__ ids) {
     using namespace cheb_mfma;
     using Sh = Shape<FN>;
     constexpr int N = Sh::N, nz = Sh::nz, pz = Sh::pz;
-    constexpr int KC = ceil_div(kPencils * nz, 256);
+    static_assert(nz <= 16 * 16 && 8 * 16 >= Sh::KSE * 32, "one 16-coefficient run per lane");
     __shared__ __align__(16) _Float16 bE[2][kCols * Sh::LDE];  // [hi, lo][column][even k / 2]
     __shared__ __align__(16) _Float16 bO[2][kCols * Sh::LDO];  // [hi, lo][column][(odd k - 1) / 2]
-    __shared__ unsigned smax[kPencils];
     __shared__ int sexp[kPencils];
     const int slot = ids[blockIdx.z / 6], field = blockIdx.z % 6;
     F += slot * sF + field * fstride;
     Y += slot * sY + field * ystride;
     const int kx = blockIdx.y, ky0 = blockIdx.x * kPencils;
     const int nb = min(kPencils, d.ny - ky0);
-    const int tid = threadIdx.x, total = nb * nz;
-    const long pstride = (long)d.hnx * nz, pin = ((long)ky0 * d.hnx + kx) * nz;
-    double2 a[KC];
-#pragma unroll
-    for (int u = 0; u < KC; u++) {
-        const int i = tid + u * 256;
-        if (i < total) {
-            const int b = i / nz, k = i - b * nz;
-            a[u] = F[pin + b * pstride + k];
-        }
-    }
-    // zero the operand tiles (padding k and ragged columns stay zero)
+    const int tid = threadIdx.x;
     {
-        uint4* z = (uint4*)&bE[0][0];
-        constexpr int nE = 2 * kCols * Sh::LDE * 2 / 16, nO = 2 * kCols * Sh::LDO * 2 / 16;
-        for (int i = tid; i < nE; i += 256) z[i] = make_uint4(0, 0, 0, 0);
-        z = (uint4*)&bO[0][0];
-        for (int i = tid; i < nO; i += 256) z[i] = make_uint4(0, 0, 0, 0);
-        if (tid < kPencils) smax[tid] = 0u;
-    }
-    __syncthreads();
+        const int b = tid >> 4, m = tid & 15, k0 = 16 * m;
+        const bool have = b < nb;
+        TI a[16];
+        const TI* src = F + ((long)(ky0 + b) * d.hnx + kx) * nz + k0;
 #pragma unroll
-    for (int u = 0; u < KC; u++) {
-        const int i = tid + u * 256;
-        if (i < total) lds_max(&smax[i / nz], (float)fmax(fabs(a[u].x), fabs(a[u].y)));
-    }
-    __syncthreads();
-    if (tid < kPencils) sexp[tid] = scale_exp(__uint_as_float(smax[tid]));
-    __syncthreads();
+        for (int i = 0; i < 16; i++) {
+            a[i].x = 0;
+            a[i].y = 0;
+            if (have && k0 + i < nz) a[i] = src[i];
+        }
+        float mx = 0.f;
 #pragma unroll
-    for (int u = 0; u < KC; u++) {
-        const int i = tid + u * 256;
-        if (i < total) {
-            const int b = i / nz, k = i - b * nz, e = sexp[b], j = k >> 1;
-            const double xr = ldexp(a[u].x, e), xi = ldexp(a[u].y, e);
-            if (k & 1) {
-                split_store(&bO[0][b * Sh::LDO + j], &bO[1][b * Sh::LDO + j], xr);
-                split_store(&bO[0][(kPencils + b) * Sh::LDO + j], &bO[1][(kPencils + b) * Sh::LDO + j], xi);
-            } else {
-                split_store(&bE[0][b * Sh::LDE + j], &bE[1][b * Sh::LDE + j], xr);
-                split_store(&bE[0][(kPencils + b) * Sh::LDE + j], &bE[1][(kPencils + b) * Sh::LDE + j], xi);
-            }
+        for (int i = 0; i < 16; i++) mx = fmaxf(mx, (float)fmax(fabs((double)a[i].x), fabs((double)a[i].y)));
+        mx = fmaxf(mx, __shfl_xor(mx, 1));
+        mx = fmaxf(mx, __shfl_xor(mx, 2));
+        mx = fmaxf(mx, __shfl_xor(mx, 4));
+        mx = fmaxf(mx, __shfl_xor(mx, 8));
+        const int e = scale_exp(mx);
+        if (m == 0) sexp[b] = e;
+        // runs j = 8 m .. 8 m + 7 of the even (k = 2 j) and odd (k = 2 j + 1) coefficients
+        h8_t erh, erl, eih, eil, orh, orl, oih, oil;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const double xr = ldexp((double)a[2 * i].x, e), xi = ldexp((double)a[2 * i].y, e);
+            const double yr = ldexp((double)a[2 * i + 1].x, e), yi = ldexp((double)a[2 * i + 1].y, e);
+            erh[i] = (_Float16)(float)xr;
+            erl[i] = (_Float16)(float)(xr - (double)(float)erh[i]);
+            eih[i] = (_Float16)(float)xi;
+            eil[i] = (_Float16)(float)(xi - (double)(float)eih[i]);
+            orh[i] = (_Float16)(float)yr;
+            orl[i] = (_Float16)(float)(yr - (double)(float)orh[i]);
+            oih[i] = (_Float16)(float)yi;
+            oil[i] = (_Float16)(float)(yi - (double)(float)oih[i]);
+        }
+        const int j0 = 8 * m;
+        if (j0 < Sh::KSE * 32) {
+            *(h8_t*)&bE[0][b * Sh::LDE + j0] = erh;
+            *(h8_t*)&bE[1][b * Sh::LDE + j0] = erl;
+            *(h8_t*)&bE[0][(kPencils + b) * Sh::LDE + j0] = eih;
+            *(h8_t*)&bE[1][(kPencils + b) * Sh::LDE + j0] = eil;
+        }
+        if (j0 < Sh::KSO * 32) {
+            *(h8_t*)&bO[0][b * Sh::LDO + j0] = orh;
+            *(h8_t*)&bO[1][b * Sh::LDO + j0] = orl;
+            *(h8_t*)&bO[0][(kPencils + b) * Sh::LDO + j0] = oih;
+            *(h8_t*)&bO[1][(kPencils + b) * Sh::LDO + j0] = oil;
         }
     }
     __syncthreads();
@@ -224,25 +234,36 @@ __global__ __launch_bounds__(256) void k_s2p_mfma(const double2* __restrict__ F,
 //     E = Q1s G- (rows p <= N/2), A = Q1a G+ (rows p < N/2);
 // Q0s[p][q] = (M0[p][q] + M0[p][2N-q])/2, Q0s[p][N] = M0[p][N], Q0a the half difference;
 // Q1a the half sums of M1 (with M1[p][N]), Q1s its half differences.
-// Block: 16 pencils (kx fixed) of one member (blockIdx.z), one wave per row tile of the
-// p <= N/2 rows (64 * RTS threads); wave w also holds row tile w of the p < N/2 rows.  The
-// inputs h3, h2, h1 are taken one at a time: fold and combine in FP32, one scale per pencil,
-// split into LDS, products.  Output: N_v, N_eta (FP64) of the slot's current history buffers.
+// Block: 16 pencils (kx fixed) of one member (blockIdx.z); two waves per row tile of the
+// p <= N/2 rows (128 * RTS threads), wave 2 rt + c holding the real (c = 0) or imaginary
+// (c = 1) parts of the row tile rt of every product: after the per-pencil combinations
+// the products act on real and imaginary parts alike.  Two stages: h3 and h2 (from f_x,
+// f_y) into two operand sets, then h1 (from f_z, loaded during the first products).  Each
+// stage: fold and combine in FP32, one power-of-two scale per pencil and input (largest
+// magnitude through LDS), split into LDS, products.  The results go through LDS to
+// coalesced rows of N_v, N_eta (FP64) in the slot's current history buffers.
 // tab: fragments of Q0s, Q0a, Q1s, Q1a (in this order); s0, s1: scales of the M0 and M1 tables.
+// LDS (dynamic): two operand sets [set][hi, lo][32 columns][LDP (G+) | LDM (G-)] halves.
 // -------------------------------------------------------------------------------------
 template <int FN>
-__global__ __launch_bounds__(64 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
+constexpr size_t p2s_mfma_lds() {
+    using Sh = cheb_mfma::Shape<FN>;
+    return sizeof(_Float16) * 2 * 2 * cheb_mfma::kCols * (Sh::LDP + Sh::LDM);
+}
+
+template <int FN>
+__global__ __launch_bounds__(128 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
     const float2* __restrict__ Z1, long zstride, long sY, double2* __restrict__ NL, long sNL, Dims d,
     const h8_t* __restrict__ tab, int s0, int s1, Members mb) {
     using namespace cheb_mfma;
     using Sh = Shape<FN>;
     constexpr int N = Sh::N, nz = Sh::nz, pz = Sh::pz;
-    constexpr int NT = 64 * Sh::RTS;               // threads
+    constexpr int NT = 128 * Sh::RTS;  // threads
     constexpr int KQ = ceil_div(kPencils * (N + 1), NT);
-    __shared__ __align__(16) _Float16 bP[2][kCols * Sh::LDP];  // G+ [hi, lo][column][q]
-    __shared__ __align__(16) _Float16 bM[2][kCols * Sh::LDM];  // G-
+    constexpr int SET = 2 * kCols * (Sh::LDP + Sh::LDM);  // halves per operand set
+    static_assert(2 * kPencils * nz * sizeof(double2) <= p2s_mfma_lds<FN>(), "output staging fits");
+    RPCF_LDS(_Float16);
     __shared__ unsigned smax[3][kPencils];
-    __shared__ int sexp[3][kPencils];
     const int mslot = mb.ids[blockIdx.z];
     const int ph = mb.phase[mslot];
     const double aphi = mb.mp[mslot].aphi, beta = mb.mp[mslot].beta;
@@ -253,14 +274,21 @@ __global__ __launch_bounds__(64 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
     const int kx = blockIdx.y, ky0 = blockIdx.x * kPencils;
     const int nb = min(kPencils, d.ny - ky0);
     const int tid = threadIdx.x, total = nb * (N + 1);
-    const int lane = tid & 63, rt = tid >> 6, lc = lane & 15, lk = 8 * (lane >> 4);
-    const h8_t* t0s = tab;
-    const h8_t* t0a = t0s + 2 * 64 * Sh::F0S;
-    const h8_t* t1s = t0a + 2 * 64 * Sh::F0A;
-    const h8_t* t1a = t1s + 2 * 64 * Sh::F1S;
+    const int lane = tid & 63, wave = tid >> 6, rt = wave >> 1, ct = wave & 1, lc = lane & 15, lk = 8 * (lane >> 4);
     const bool anti = rt < Sh::RTA;
-    if (tid < 3 * kPencils) (&smax[0][0])[tid] = 0u;
-    // pencil scalars of an item's pencil b: i alpha, i beta, k^2; pencil (0,0) flagged
+    const bool has_mean = kx == 0 && ky0 == 0;
+    // operand tiles of set s: G+ hi, G+ lo, G- hi, G- lo
+    auto gp_hi = [&](int st) { return lds + st * SET; };
+    auto gp_lo = [&](int st) { return lds + st * SET + kCols * Sh::LDP; };
+    auto gm_hi = [&](int st) { return lds + st * SET + 2 * kCols * Sh::LDP; };
+    auto gm_lo = [&](int st) { return lds + st * SET + 2 * kCols * Sh::LDP + kCols * Sh::LDM; };
+    {
+        // zero both sets once: padding k and ragged columns stay zero, the rest is overwritten
+        uint4* z = (uint4*)lds;
+        for (int i = tid; i < 2 * SET * 2 / 16; i += NT) z[i] = make_uint4(0, 0, 0, 0);
+        if (tid < 3 * kPencils) (&smax[0][0])[tid] = 0u;
+    }
+    // pencil scalars of pencil b: alpha, beta, k^2; true for pencil (0,0)
     auto scal = [&](int b, float* ia, float* ib, float* k2) {
         const int ky = ky0 + b;
         const double a_ = (double)kx / aphi;
@@ -268,9 +296,9 @@ __global__ __launch_bounds__(64 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
         *ia = (float)a_;
         *ib = (float)b_;
         *k2 = (float)(a_ * a_ + b_ * b_);
-        return kx == 0 && ky == 0;
+        return has_mean && b == 0;
     };
-    // samples q and 2N - q of one component for every item (q = i / nb, b = i % nb)
+    // samples q and 2N - q of component g for every item (q = i / nb, b = i % nb)
     auto load = [&](int g, float2* fa, float2* fb) {
         const float2* in = Z1 + g * zstride + (long)kx * pz * d.ny + ky0;
 #pragma unroll
@@ -283,139 +311,176 @@ __global__ __launch_bounds__(64 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
             }
         }
     };
-    // accumulators (re, im tiles): N_eta sym/anti via M0, N_v sym/anti via M0 and via M1
-    f4_t es[2], ea[2], v0s[2], v0a[2], v1s[2], v1a[2];
+    auto fold = [&](float2 a, float2 b, int q, float2* gp, float2* gm) {
+        *gp = q < N ? make_float2(a.x + b.x, a.y + b.y) : a;
+        *gm = q < N ? make_float2(a.x - b.x, a.y - b.y) : make_float2(0.f, 0.f);
+    };
+    auto mag = [](float2 a, float2 b) { return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(b.x), fabsf(b.y))); };
+    // largest magnitude per pencil of input h over the block: with nb == 16 every item of a
+    // thread is pencil tid % 16, so a thread's maximum, then the lanes of its pencil in the
+    // wave (xor 16, 32), then one atomic per wave and pencil
+    auto pencil_max = [&](int h, const float* m) {
+        if (nb == kPencils) {
+            float v = 0.f;
 #pragma unroll
-    for (int c = 0; c < 2; c++) es[c] = ea[c] = v0s[c] = v0a[c] = v1s[c] = v1a[c] = f4_t{0, 0, 0, 0};
-    // one input h: hp, hm its folded G+, G- per item; products sym (table ts, K = ks blocks of
-    // the G+ tile if sym_plus) and anti into the given accumulators
-    auto stage = [&](int hidx, const float2* hp, const float2* hm, const h8_t* ts, const h8_t* ta, bool sym_plus,
-                     f4_t* accs, f4_t* acca) {
-        {
-            uint4* z = (uint4*)&bP[0][0];
-            constexpr int nP = 2 * kCols * Sh::LDP * 2 / 16, nM = 2 * kCols * Sh::LDM * 2 / 16;
-            for (int i = tid; i < nP; i += NT) z[i] = make_uint4(0, 0, 0, 0);
-            z = (uint4*)&bM[0][0];
-            for (int i = tid; i < nM; i += NT) z[i] = make_uint4(0, 0, 0, 0);
-        }
+            for (int u = 0; u < KQ; u++) v = fmaxf(v, m[u]);
+            v = fmaxf(v, __shfl_xor(v, 16));
+            v = fmaxf(v, __shfl_xor(v, 32));
+            if (lane < 16) lds_max(&smax[h][lane], v);
+        } else {
 #pragma unroll
-        for (int u = 0; u < KQ; u++) {
-            const int i = tid + u * NT;
-            if (i < total) {
-                const int b = i % nb;
-                lds_max(&smax[hidx][b], fmaxf(fmaxf(fabsf(hp[u].x), fabsf(hp[u].y)), fmaxf(fabsf(hm[u].x), fabsf(hm[u].y))));
+            for (int u = 0; u < KQ; u++) {
+                const int i = tid + u * NT;
+                if (i < total) lds_max(&smax[h][i % nb], m[u]);
             }
         }
-        __syncthreads();
-        if (tid < kPencils) sexp[hidx][tid] = scale_exp(__uint_as_float(smax[hidx][tid]));
-        __syncthreads();
+    };
+    auto split_set = [&](int st, int h, const float2* hp, const float2* hm) {
 #pragma unroll
         for (int u = 0; u < KQ; u++) {
             const int i = tid + u * NT;
             if (i < total) {
-                const int q = i / nb, b = i - q * nb, e = sexp[hidx][b];
-                split_store(&bP[0][b * Sh::LDP + q], &bP[1][b * Sh::LDP + q], ldexpf(hp[u].x, e));
-                split_store(&bP[0][(kPencils + b) * Sh::LDP + q], &bP[1][(kPencils + b) * Sh::LDP + q], ldexpf(hp[u].y, e));
+                const int q = i / nb, b = i - q * nb, e = scale_exp(__uint_as_float(smax[h][b]));
+                split_store(&gp_hi(st)[b * Sh::LDP + q], &gp_lo(st)[b * Sh::LDP + q], ldexpf(hp[u].x, e));
+                split_store(&gp_hi(st)[(kPencils + b) * Sh::LDP + q], &gp_lo(st)[(kPencils + b) * Sh::LDP + q],
+                            ldexpf(hp[u].y, e));
                 if (q < N) {
-                    split_store(&bM[0][b * Sh::LDM + q], &bM[1][b * Sh::LDM + q], ldexpf(hm[u].x, e));
-                    split_store(&bM[0][(kPencils + b) * Sh::LDM + q], &bM[1][(kPencils + b) * Sh::LDM + q],
+                    split_store(&gm_hi(st)[b * Sh::LDM + q], &gm_lo(st)[b * Sh::LDM + q], ldexpf(hm[u].x, e));
+                    split_store(&gm_hi(st)[(kPencils + b) * Sh::LDM + q], &gm_lo(st)[(kPencils + b) * Sh::LDM + q],
                                 ldexpf(hm[u].y, e));
                 }
             }
         }
-        __syncthreads();
-        auto prod = [&](const h8_t* t, int ks, const _Float16* bh, const _Float16* bl, int ld, f4_t* acc) {
-            for (int s = 0; s < ks; s++) {
-                const h8_t ah = t[((rt * ks + s) * 2 + 0) * 64 + lane];
-                const h8_t al = t[((rt * ks + s) * 2 + 1) * 64 + lane];
-                const int o = s * 32 + lk;
-                acc[0] = mma3(ah, al, *(const h8_t*)&bh[lc * ld + o], *(const h8_t*)&bl[lc * ld + o], acc[0]);
-                acc[1] = mma3(ah, al, *(const h8_t*)&bh[(16 + lc) * ld + o], *(const h8_t*)&bl[(16 + lc) * ld + o], acc[1]);
-            }
-        };
-        // sym rows from G+ (M0) or G- (M1); anti rows from the other one
-        if (sym_plus) prod(ts, Sh::KSP, bP[0], bP[1], Sh::LDP, accs);
-        else prod(ts, Sh::KSM, bM[0], bM[1], Sh::LDM, accs);
-        if (anti) {
-            if (sym_plus) prod(ta, Sh::KSM, bM[0], bM[1], Sh::LDM, acca);
-            else prod(ta, Sh::KSP, bP[0], bP[1], Sh::LDP, acca);
-        }
-        __syncthreads();
     };
-    float2 xa[KQ], xb[KQ], ya[KQ], yb[KQ];
-    __syncthreads();  // smax cleared
-    load(0, xa, xb);
-    load(1, ya, yb);
+    // acc += table (row tile rt, ks k-steps) x operand columns of this wave's part ct
+    auto prod = [&](const h8_t* t, int ks, const _Float16* bh, const _Float16* bl, int ld, f4_t acc) {
+        const int col = ct * kPencils + lc;
+#pragma unroll
+        for (int s = 0; s < ks; s++) {
+            const h8_t ah = t[((rt * ks + s) * 2 + 0) * 64 + lane];
+            const h8_t al = t[((rt * ks + s) * 2 + 1) * 64 + lane];
+            const int o = col * ld + s * 32 + lk;
+            acc = mma3(ah, al, *(const h8_t*)&bh[o], *(const h8_t*)&bl[o], acc);
+        }
+        return acc;
+    };
+    const h8_t* t0s = tab;
+    const h8_t* t0a = t0s + 2 * 64 * Sh::F0S;
+    const h8_t* t1s = t0a + 2 * 64 * Sh::F0A;
+    const h8_t* t1a = t1s + 2 * 64 * Sh::F1S;
+    const f4_t zero4 = {0, 0, 0, 0};
+    f4_t es = zero4, ea = zero4, v1s = zero4, v1a = zero4, v0s = zero4, v0a = zero4;
+    float2 za[KQ], zb[KQ];
+    __syncthreads();  // tiles zeroed, maxima cleared
     {
-        // h3 = i alpha f_y - i beta f_x (pencil (0,0): f_y);  h2 = i alpha f_x + i beta f_y (0)
-        float2 hp[KQ], hm[KQ];
+        // stage 1: h3 = i alpha f_y - i beta f_x (pencil (0,0): f_y) -> set 0;
+        //          h2 = i alpha f_x + i beta f_y (pencil (0,0): 0)   -> set 1
+        float2 h3p[KQ], h3m[KQ], h2p[KQ], h2m[KQ];
+        {
+            float2 xa[KQ], xb[KQ], ya[KQ], yb[KQ];
+            load(0, xa, xb);
+            load(1, ya, yb);
+            float m3[KQ], m2[KQ];
+#pragma unroll
+            for (int u = 0; u < KQ; u++) {
+                m3[u] = m2[u] = 0.f;
+                const int i = tid + u * NT;
+                if (i < total) {
+                    const int q = i / nb, b = i - q * nb;
+                    float ia, ib, k2;
+                    const bool mean = scal(b, &ia, &ib, &k2);
+                    float2 xp, xm, yp, ym;
+                    fold(xa[u], xb[u], q, &xp, &xm);
+                    fold(ya[u], yb[u], q, &yp, &ym);
+                    // i a y - i b x = (-a y.im + b x.im, a y.re - b x.re)
+                    h3p[u] = mean ? yp : make_float2(-ia * yp.y + ib * xp.y, ia * yp.x - ib * xp.x);
+                    h3m[u] = mean ? ym : make_float2(-ia * ym.y + ib * xm.y, ia * ym.x - ib * xm.x);
+                    // i a x + i b y = (-a x.im - b y.im, a x.re + b y.re)
+                    h2p[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xp.y - ib * yp.y, ia * xp.x + ib * yp.x);
+                    h2m[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xm.y - ib * ym.y, ia * xm.x + ib * ym.x);
+                    m3[u] = mag(h3p[u], h3m[u]);
+                    m2[u] = mag(h2p[u], h2m[u]);
+                }
+            }
+            pencil_max(0, m3);
+            pencil_max(1, m2);
+        }
+        __syncthreads();
+        split_set(0, 0, h3p, h3m);
+        split_set(1, 1, h2p, h2m);
+    }
+    load(2, za, zb);  // f_z for h1, in flight during the first products
+    __syncthreads();
+    // N_eta: sym rows Q0s G+, anti rows Q0a G- of h3 (set 0); N_v (M1 part): sym rows Q1s G-,
+    // anti rows Q1a G+ of h2 (set 1)
+    es = prod(t0s, Sh::KSP, gp_hi(0), gp_lo(0), Sh::LDP, es);
+    v1s = prod(t1s, Sh::KSM, gm_hi(1), gm_lo(1), Sh::LDM, v1s);
+    if (anti) {
+        ea = prod(t0a, Sh::KSM, gm_hi(0), gm_lo(0), Sh::LDM, ea);
+        v1a = prod(t1a, Sh::KSP, gp_hi(1), gp_lo(1), Sh::LDP, v1a);
+    }
+    {
+        // stage 2: h1 = k^2 f_z (pencil (0,0): f_x) -> set 0
+        float2 h1p[KQ], h1m[KQ];
+        float m1[KQ];
+        const float2* x0 = Z1 + (long)kx * pz * d.ny + ky0;  // f_x of pencil (0,0)
 #pragma unroll
         for (int u = 0; u < KQ; u++) {
+            m1[u] = 0.f;
             const int i = tid + u * NT;
             if (i < total) {
                 const int q = i / nb, b = i - q * nb;
                 float ia, ib, k2;
                 const bool mean = scal(b, &ia, &ib, &k2);
-                const float2 xp = q < N ? make_float2(xa[u].x + xb[u].x, xa[u].y + xb[u].y) : xa[u];
-                const float2 xm = q < N ? make_float2(xa[u].x - xb[u].x, xa[u].y - xb[u].y) : make_float2(0.f, 0.f);
-                const float2 yp = q < N ? make_float2(ya[u].x + yb[u].x, ya[u].y + yb[u].y) : ya[u];
-                const float2 ym = q < N ? make_float2(ya[u].x - yb[u].x, ya[u].y - yb[u].y) : make_float2(0.f, 0.f);
-                // i a y - i b x = (-a y.im + b x.im, a y.re - b x.re)
-                hp[u] = mean ? yp : make_float2(-ia * yp.y + ib * xp.y, ia * yp.x - ib * xp.x);
-                hm[u] = mean ? ym : make_float2(-ia * ym.y + ib * xm.y, ia * ym.x - ib * xm.x);
-                // keep h2 in the x registers: i a x + i b y = (-a x.im - b y.im, a x.re + b y.re)
-                xa[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xp.y - ib * yp.y, ia * xp.x + ib * yp.x);
-                xb[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xm.y - ib * ym.y, ia * xm.x + ib * ym.x);
+                float2 zp, zm;
+                if (mean) fold(x0[(long)q * d.ny], x0[(long)(FN - q) * d.ny], q, &zp, &zm);
+                else fold(za[u], zb[u], q, &zp, &zm);
+                h1p[u] = mean ? zp : make_float2(k2 * zp.x, k2 * zp.y);
+                h1m[u] = mean ? zm : make_float2(k2 * zm.x, k2 * zm.y);
+                m1[u] = mag(h1p[u], h1m[u]);
             }
         }
-        load(2, ya, yb);  // f_z for h1
-        stage(0, hp, hm, t0s, t0a, true, es, ea);
+        pencil_max(2, m1);
+        __syncthreads();  // also: set 0 is no longer read by the stage-1 products
+        split_set(0, 2, h1p, h1m);
     }
-    stage(1, xa, xb, t1s, t1a, false, v1s, v1a);
-    {
-        // h1 = k^2 f_z (pencil (0,0): f_x, reloaded)
-        const bool has_mean = kx == 0 && ky0 == 0;
-        if (has_mean) load(0, xa, xb);
-        float2 hp[KQ], hm[KQ];
-#pragma unroll
-        for (int u = 0; u < KQ; u++) {
-            const int i = tid + u * NT;
-            if (i < total) {
-                const int q = i / nb, b = i - q * nb;
-                float ia, ib, k2;
-                const bool mean = scal(b, &ia, &ib, &k2);
-                const float2 fa = mean ? xa[u] : ya[u], fb = mean ? xb[u] : yb[u];
-                const float2 zp = q < N ? make_float2(fa.x + fb.x, fa.y + fb.y) : fa;
-                const float2 zm = q < N ? make_float2(fa.x - fb.x, fa.y - fb.y) : make_float2(0.f, 0.f);
-                hp[u] = mean ? zp : make_float2(k2 * zp.x, k2 * zp.y);
-                hm[u] = mean ? zm : make_float2(k2 * zm.x, k2 * zm.y);
-            }
-        }
-        stage(2, hp, hm, t0s, t0a, true, v0s, v0a);
-    }
-    // combine per pencil: lane holds pencil b = lc, rows 16 rt + 4 (lane / 16) + r
+    __syncthreads();
+    v0s = prod(t0s, Sh::KSP, gp_hi(0), gp_lo(0), Sh::LDP, v0s);
+    if (anti) v0a = prod(t0a, Sh::KSM, gm_hi(0), gm_lo(0), Sh::LDM, v0a);
+    // combine: lane holds part ct of pencil b = lc, rows 16 rt + 4 (lane / 16) + r; results go
+    // to LDS as [N_v, N_eta][pencil][row] complex doubles, then out in rows of nz
     const int b = lc;
-    if (b >= nb) return;
-    const double se = ldexp(1.0, -sexp[0][b] - s0), sv1 = ldexp(1.0, -sexp[1][b] - s1),
-                 sv0 = ldexp(1.0, -sexp[2][b] - s0);
-    const long po = ((long)(ky0 + b) * d.hnx + kx) * nz;
+    const float e3 = (float)__uint_as_float(smax[0][b]), e2 = __uint_as_float(smax[1][b]), e1 = __uint_as_float(smax[2][b]);
+    const double se = ldexp(1.0, -scale_exp(e3) - s0), sv1 = ldexp(1.0, -scale_exp(e2) - s1),
+                 sv0 = ldexp(1.0, -scale_exp(e1) - s0);
+    __syncthreads();  // operand sets no longer read: the output staging reuses them
+    double* outv = (double*)lds;                        // [pencil][nz][2]
+    double* oute = outv + 2 * kPencils * nz;
+    if (b < nb) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int p = rt * 16 + (lane >> 4) * 4 + r;
-        if (2 * p > N) continue;
+        for (int r = 0; r < 4; r++) {
+            const int p = rt * 16 + (lane >> 4) * 4 + r;
+            if (2 * p > N) continue;
 #pragma unroll
-        for (int side = 0; side < 2; side++) {
-            if (side == 1 && 2 * p == N) continue;
-            const double sg = side ? -1.0 : 1.0;
-            auto val = [&](const f4_t* s_, const f4_t* a_, int c) {
-                return (double)s_[c][r] + (anti ? sg * (double)a_[c][r] : 0.0);
-            };
-            const double2 ne = make_double2(val(es, ea, 0) * se, val(es, ea, 1) * se);
-            const double2 nv = make_double2(val(v0s, v0a, 0) * sv0 + val(v1s, v1a, 0) * sv1,
-                                            val(v0s, v0a, 1) * sv0 + val(v1s, v1a, 1) * sv1);
-            const int row = side ? N - p : p;
-            NLv[po + row] = nv;
-            NLe[po + row] = ne;
+            for (int side = 0; side < 2; side++) {
+                if (side == 1 && 2 * p == N) continue;
+                const double sg = side ? -1.0 : 1.0;
+                const double ev = (double)es[r] + (anti ? sg * (double)ea[r] : 0.0);
+                const double vv0 = (double)v0s[r] + (anti ? sg * (double)v0a[r] : 0.0);
+                const double vv1 = (double)v1s[r] + (anti ? sg * (double)v1a[r] : 0.0);
+                const int row = side ? N - p : p;
+                oute[(b * nz + row) * 2 + ct] = ev * se;
+                outv[(b * nz + row) * 2 + ct] = vv0 * sv0 + vv1 * sv1;
+            }
         }
+    }
+    __syncthreads();
+    const double2* ov = (const double2*)outv;
+    const double2* oe = (const double2*)oute;
+    const long po = ((long)ky0 * d.hnx + kx) * nz, pstride = (long)d.hnx * nz;
+    for (int i = tid; i < nb * nz; i += NT) {
+        const int bb = i / nz, row = i - bb * nz;
+        NLv[po + bb * pstride + row] = ov[i];
+        NLe[po + bb * pstride + row] = oe[i];
     }
 }

@@ -709,9 +709,9 @@ __device__ __forceinline__ void cheb_deriv_scan(double2* fb, int ld, int nz, int
 //                   ox = D w_span - i b v,  oy = -D u + i a v,  oz = eta;
 // pencil (0,0):     u = u0, w_span = w0, v = 0, ox = D w0, oy = -D u0, oz = 0.
 // -------------------------------------------------------------------------------------
-template <int FN>
+template <int FN, typename TO = double2>  // TO: storage of the six fields (FP32 for the mixed form's MFMA path)
 __global__ __launch_bounds__(256) void k_velocity_tile(const double2* __restrict__ sv, const double2* __restrict__ se,
-                                                       long sS, double2* __restrict__ F, long fstride, long sF, Dims d,
+                                                       long sS, TO* __restrict__ F, long fstride, long sF, Dims d,
                                                        Members mb, int TB, int ld) {
     static_assert(FN > 0, "compiled lengths only");
     RPCF_LDS(double2);
@@ -815,12 +815,12 @@ __global__ __launch_bounds__(256) void k_velocity_tile(const double2* __restrict
                 oy = make_double2(du.x * -1.0 + (0.0 * vv.x - ia * vv.y), du.y * -1.0 + (0.0 * vv.y + ia * vv.x));
             }
             const long o = pin + b * pstride + k;
-            F[0 * fstride + o] = su[u];
-            F[1 * fstride + o] = sw[u];
-            F[2 * fstride + o] = vv;
-            F[3 * fstride + o] = ox;
-            F[4 * fstride + o] = oy;
-            F[5 * fstride + o] = ee;
+            F[0 * fstride + o] = to_c<TO>(su[u]);
+            F[1 * fstride + o] = to_c<TO>(sw[u]);
+            F[2 * fstride + o] = to_c<TO>(vv);
+            F[3 * fstride + o] = to_c<TO>(ox);
+            F[4 * fstride + o] = to_c<TO>(oy);
+            F[5 * fstride + o] = to_c<TO>(ee);
         }
     }
 }

@@ -83,6 +83,7 @@ struct rpcf_ctx {
     float* twf = nullptr;
     int cap_x = 0, cap_y = 0, cap_zp = 0, cap_zn = 0;  // fft_capacity of each plan
     int tv_rt = 0, tv_s = 0;                           // MFMA row tiles and k-steps of the operator products
+    int tv_items = 0;                                  // operator products by k_lin_tv_items (0: k_lin_tv_mfma)
     // mixed form: Chebyshev transforms as split-FP16 products (k_cheb_mfma.hpp)
     int use_mfma = 0;
     _Float16* MT = nullptr;  // s2p fragments, then p2s fragments
@@ -334,6 +335,38 @@ int download_spec(rpcf_ctx* c, double* dst, const double2* src, long stride) {
     return RPCF_OK;
 }
 
+// The six fields of the mixed form's matrix-core path are stored in FP32 (complex float,
+// same element indexing): these move one field across the boundary with the conversion.
+int upload_spec_f32(rpcf_ctx* c, float2* dst, const double* src, long stride) {
+    if (!src || stride < c->d.nz) return fail(c, RPCF_EINVAL, "bad spectral array or stride");
+    const int nz = c->d.nz;
+    std::vector<float2> h(spec_elems(c));
+    for (size_t p = 0; p < (size_t)c->d.P; p++)
+        for (int k = 0; k < nz; k++) {
+            const double* s = src + 2 * ((size_t)stride * p + k);
+            h[p * nz + k] = make_float2((float)s[0], (float)s[1]);
+        }
+    HIPCHK(c, hipMemcpyAsync(dst, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPCF_OK;
+}
+
+int download_spec_f32(rpcf_ctx* c, double* dst, const float2* src, long stride) {
+    if (!dst || stride < c->d.nz) return fail(c, RPCF_EINVAL, "bad spectral array or stride");
+    const int nz = c->d.nz;
+    std::vector<float2> h(spec_elems(c));
+    int rc = sync_ctx(c);
+    if (rc) return rc;
+    RCCHK(copy_sync(c, h.data(), src, sizeof(float2) * h.size(), hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < (size_t)c->d.P; p++)
+        for (int k = 0; k < nz; k++) {
+            double* s = dst + 2 * ((size_t)stride * p + k);
+            s[0] = h[p * nz + k].x;
+            s[1] = h[p * nz + k].y;
+        }
+    return RPCF_OK;
+}
+
 // history buffer of a slot: which = 0 N_v, 1 N_eta; prev selects N^{n-1}
 double2* hist(rpcf_ctx* c, int slot, int which, int prev) {
     const int ph = c->slots[slot].phase;
@@ -376,7 +409,15 @@ int enq_velocity(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
             dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
             const size_t lds = sizeof(double2) * ((size_t)TB * ld + 256);
             switch (2 * d.N) {
-#define RPCF_CASE(L) case L: k_velocity_tile<L><<<g, kFftThreads, lds, c->stream>>>(c->SV, c->SE, c->sS, c->F, fS, c->sF, d, members(c, ids), TB, ld); break;
+#define RPCF_CASE(L)                                                                                          \
+    case L:                                                                                                   \
+        if (c->use_mfma)                                                                                      \
+            k_velocity_tile<L, float2><<<g, kFftThreads, lds, c->stream>>>(c->SV, c->SE, c->sS, (float2*)c->F, fS, \
+                                                                           c->sF, d, members(c, ids), TB, ld);   \
+        else                                                                                                  \
+            k_velocity_tile<L><<<g, kFftThreads, lds, c->stream>>>(c->SV, c->SE, c->sS, c->F, fS, c->sF, d,        \
+                                                                   members(c, ids), TB, ld);                    \
+        break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                 default: {
@@ -422,7 +463,8 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
 #define RPCF_CASE(L)                                                                                               \
     case L:                                                                                                        \
         if constexpr (L <= 384)                                                                                    \
-            k_s2p_mfma<(L <= 384 ? L : 384)><<<gm, 256, 0, c->stream>>>(c->F, fS, c->sF, Y, yS, c->sY, d,          \
+            k_s2p_mfma<(L <= 384 ? L : 384), float2><<<gm, 256, 0, c->stream>>>((const float2*)c->F, fS, c->sF, Y,  \
+                                                                               yS, c->sY, d,                       \
                                                                        (const h8_t*)c->MT, ids);                  \
         break;
                             RPCF_FFT_LENGTHS(RPCF_CASE)
@@ -524,7 +566,8 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
 #define RPCF_CASE(L)                                                                                                 \
     case L:                                                                                                          \
         if constexpr (L <= 384)                                                                                      \
-            k_p2s_mfma<(L <= 384 ? L : 384)><<<g, 64 * cheb_mfma::Shape<(L <= 384 ? L : 384)>::RTS, 0, c->stream>>>( \
+            k_p2s_mfma<(L <= 384 ? L : 384)><<<g, 128 * cheb_mfma::Shape<(L <= 384 ? L : 384)>::RTS,               \
+                                             p2s_mfma_lds<(L <= 384 ? L : 384)>(), c->stream>>>(                    \
                 Y, yS, c->sY, c->NL, c->sNL, d, (const h8_t*)(c->MT + c->mt_p2s), c->mt_s0, c->mt_s1, mb);          \
         break;
                         RPCF_FFT_LENGTHS(RPCF_CASE)
@@ -588,8 +631,12 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
             switch (2 * d.N) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
-        k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, \
-                                                                   c->sTf, c->tv_rt, ids);                      \
+        if (c->tv_items)                                                                                         \
+            k_lin_tv_items<(L / 4 + 4) / 4><<<dim3((d.P + 7) / 8, n), 256, 0, c->stream>>>(                       \
+                c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, c->sTf, c->tv_rt, ids);                            \
+        else                                                                                                     \
+            k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d,    \
+                                                                       c->TF, c->sTf, c->tv_rt, ids);           \
         break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
@@ -873,6 +920,10 @@ int init_ctx(rpcf_ctx* c) {
     if ((rc = make_plan(c, 4 * d.N, tw, &c->pl_zp, &c->cap_zp))) return rc;
     tw += 2 * 4 * d.N;
     if ((rc = make_plan(c, 2 * d.N, tw, &c->pl_zn, &c->cap_zn))) return rc;
+    {
+        const char* e = getenv("RPCF_TV_ITEMS");
+        c->tv_items = e && e[0] == '1';
+    }
     // mixed form: fragments of the split-FP16 Chebyshev products (k_cheb_mfma.hpp) for the
     // compiled lengths up to 2N = 384; RPCF_MIXED_MFMA=0 keeps the FFT-based kernels
     if (c->mixed && 2 * d.N <= 384) {
@@ -959,7 +1010,9 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_cheb_s2p_h<(L >= 64 ? L : 64)>, lim);      \
     allow_lds(k_cheb_s2p_h<(L >= 64 ? L : 64), float>, lim); \
     allow_lds(k_p2s_nl2<L, float>, lim);                  \
-    allow_lds(k_velocity_tile<L>, lim);
+    allow_lds(k_velocity_tile<L>, lim);                   \
+    allow_lds(k_velocity_tile<L, float2>, lim);           \
+    allow_lds(k_p2s_mfma<(L <= 384 ? L : 384)>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     allow_lds(k_cheb_p2s, lim);
@@ -1270,7 +1323,8 @@ int rpcf_set_field(rpcf_ctx* c, int field, const double* f_hat, long stride) {
         // the other five fields keep their values: materialise them first
         if ((rc = enq_velocity(c, c->ids_zero, 1))) return rc;
     }
-    rc = upload_spec(c, c->F + (size_t)field * spec_elems(c), f_hat, stride);
+    rc = c->use_mfma ? upload_spec_f32(c, (float2*)c->F + (size_t)field * spec_elems(c), f_hat, stride)
+                     : upload_spec(c, c->F + (size_t)field * spec_elems(c), f_hat, stride);
     if (!rc) {
         sh.have_state = 1;
         sh.fields = FIELDS_EXPLICIT;
@@ -1287,7 +1341,8 @@ int rpcf_get_field(rpcf_ctx* c, int field, double* f_hat, long stride) {
         if ((rc = enq_velocity(c, c->ids_zero, 1))) return rc;
         sh.fields = FIELDS_FRESH;
     }
-    return download_spec(c, f_hat, c->F + (size_t)field * spec_elems(c), stride);
+    return c->use_mfma ? download_spec_f32(c, f_hat, (const float2*)c->F + (size_t)field * spec_elems(c), stride)
+                       : download_spec(c, f_hat, c->F + (size_t)field * spec_elems(c), stride);
 }
 
 int rpcf_get_nonlinear(rpcf_ctx* c, int prev, double* n_v, double* n_eta, long stride) {
