@@ -460,67 +460,6 @@ __global__ __launch_bounds__(256, tv_waves(S)) void k_lin_tv_mfma(const double2*
     }
 }
 
-// The same products with the work of 8 pencils spread over the four waves of a block: the
-// 5 RT (operator, row tile) items of the 16 columns (8 pencils, real and imaginary parts)
-// go to wave w = item % 4, the v-operator items (T0, T2, T4q) first, then the eta ones
-// (T0, T2q); a wave loads the even and odd coefficients of the field its next item needs
-// into registers and reads the A fragments straight from the tile table (L2-resident, one
-// 16x4 fragment per lane and k-step), so no LDS staging, no barrier, and about four waves
-// per SIMD at 128x129x128 (1040 blocks).  Same tiles, same summation order as
-// k_lin_tv_mfma: bit-identical results.
-template <int S>
-__global__ __launch_bounds__(256, S <= 17 ? 2 : 1) void k_lin_tv_items(const double2* __restrict__ sv, const double2* __restrict__ se,
-                                                      long sS, double2* __restrict__ W, long sWl, Dims d,
-                                                      const double* __restrict__ Tf, long sTf, int RT,
-                                                      const int* __restrict__ ids) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    {
-        const long slot = ids[blockIdx.y];
-        sv += slot * sS;
-        se += slot * sS;
-        W += slot * sWl;
-        Tf += slot * sTf;
-    }
-    const int n = d.nz, N = d.N, col = lane & 15, kq = lane >> 4;
-    const int p = blockIdx.x * 8 + (col >> 1);
-    const bool okp = p < d.P;
-    const int rows = N / 2 + 1;
-    double be[S], bo[S];
-    int loaded = -1;  // field whose coefficients are in be, bo
-    for (int item = wave; item < 5 * RT; item += 4) {
-        const int m = item / RT, t = item - m * RT, fld = m < 3 ? 0 : 1;
-        if (fld != loaded) {
-            const double* xp = (const double*)((fld ? se : sv) + (long)(okp ? p : 0) * n) + (col & 1);
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                const int je = 2 * (4 * s + kq), jo = je + 1;
-                be[s] = xp[2 * min(je, n - 1)] * ((okp && je < n) ? 1.0 : 0.0);
-                bo[s] = xp[2 * min(jo, n - 1)] * ((okp && jo < n) ? 1.0 : 0.0);
-            }
-            loaded = fld;
-        }
-        const double* Ae = Tf + (((long)m * 2 + 0) * RT + t) * S * 64 + lane;
-        const double* Ao = Tf + (((long)m * 2 + 1) * RT + t) * S * 64 + lane;
-        rpcf_v4d ae = {0.0, 0.0, 0.0, 0.0}, ao = ae;
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            ae = __builtin_amdgcn_mfma_f64_16x16x4f64(Ae[s * 64], be[s], ae, 0, 0, 0);
-            ao = __builtin_amdgcn_mfma_f64_16x16x4f64(Ao[s * 64], bo[s], ao, 0, 0, 0);
-        }
-        if (okp) {
-            double* Wm = (double*)(W + (long)m * d.P * n) + (long)p * 2 * n + (col & 1);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = 16 * t + kq + 4 * r;
-                if (row < rows) {
-                    Wm[2 * row] = ae[r] + ao[r];
-                    if (N - row != row) Wm[2 * (N - row)] = ae[r] - ao[r];
-                }
-            }
-        }
-    }
-}
-
 // y_q = M x_q for the two vectors of a slot with every thread of the block: G = nthr / n
 // groups of n threads (thread t = g*n + r owns row r), group g takes the columns
 // j = g, g + G, g + 2G, .. (fused multiply-adds, j ascending), so that the block reads the

@@ -83,7 +83,6 @@ struct rpcf_ctx {
     float* twf = nullptr;
     int cap_x = 0, cap_y = 0, cap_zp = 0, cap_zn = 0;  // fft_capacity of each plan
     int tv_rt = 0, tv_s = 0;                           // MFMA row tiles and k-steps of the operator products
-    int tv_items = 0;                                  // operator products by k_lin_tv_items (0: k_lin_tv_mfma)
     // mixed form: Chebyshev transforms as split-FP16 products (k_cheb_mfma.hpp)
     int use_mfma = 0;
     _Float16* MT = nullptr;  // s2p fragments, then p2s fragments
@@ -631,12 +630,8 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
             switch (2 * d.N) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
-        if (c->tv_items)                                                                                         \
-            k_lin_tv_items<(L / 4 + 4) / 4><<<dim3((d.P + 7) / 8, n), 256, 0, c->stream>>>(                       \
-                c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, c->sTf, c->tv_rt, ids);                            \
-        else                                                                                                     \
-            k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d,    \
-                                                                       c->TF, c->sTf, c->tv_rt, ids);           \
+        k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, \
+                                                                   c->sTf, c->tv_rt, ids);                      \
         break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
@@ -920,10 +915,6 @@ int init_ctx(rpcf_ctx* c) {
     if ((rc = make_plan(c, 4 * d.N, tw, &c->pl_zp, &c->cap_zp))) return rc;
     tw += 2 * 4 * d.N;
     if ((rc = make_plan(c, 2 * d.N, tw, &c->pl_zn, &c->cap_zn))) return rc;
-    {
-        const char* e = getenv("RPCF_TV_ITEMS");
-        c->tv_items = e && e[0] == '1';
-    }
     // mixed form: fragments of the split-FP16 Chebyshev products (k_cheb_mfma.hpp) for the
     // compiled lengths up to 2N = 384; RPCF_MIXED_MFMA=0 keeps the FFT-based kernels
     if (c->mixed && 2 * d.N <= 384) {
