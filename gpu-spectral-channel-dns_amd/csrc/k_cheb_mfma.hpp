@@ -53,7 +53,9 @@ struct Shape {
     // negated), against G+ (N + 1 entries: sums of mirrored samples and the centre) or G- (N)
     static constexpr int RTS = ceil_div(N / 2 + 1, 16), RTA = ceil_div(N / 2, 16);
     static constexpr int KSP = ceil_div(N + 1, 32), KSM = ceil_div(N, 32);
-    static constexpr int LDP = KSP * 32 + 8, LDM = KSM * 32 + 8;
+    // LDS row strides of the p2s operand tiles: padded by 8 halves against bank conflicts,
+    // except at 2N = 384, where three operand sets would not fit in 160 KB with the padding
+    static constexpr int LDP = KSP * 32 + (FN >= 384 ? 0 : 8), LDM = KSM * 32 + (FN >= 384 ? 0 : 8);
     // fragments of each table, in the order they are stored
     static constexpr int FE = RTE * KSE, FO = RTO * KSO;
     static constexpr int F0S = RTS * KSP, F0A = RTA * KSM, F1S = RTS * KSM, F1A = RTA * KSP;
@@ -88,130 +90,183 @@ __device__ __forceinline__ void lds_max(unsigned* slot, float v) { atomicMax(slo
 }  // namespace cheb_mfma
 
 // -------------------------------------------------------------------------------------
-// Padded s2p of the six spectral fields (mixed form): F[f][P][nz] (FP64, or FP32 when the
-// velocity recovery stores it so) -> Y[f][kx][q][ky] (FP32), q = 0..2N.
+// Padded s2p of the six spectral fields (mixed form): F[f][P][nz] (FP32, as the velocity
+// recovery of this form stores them) -> Y[f][kx][q][ky] (FP32), q = 0..2N.
 // f_q = sum_k a_k cos(pi k q / (2N)); with E_q = sum_{k even} a_k cos(pi k q/(2N)) (q <= N)
 // and O_q the odd sum (q < N): f_q = E_q + O_q, f_{2N-q} = E_q - O_q, f_N = E_N.
-// Block: 16 pencils (ky0.., kx fixed) of one field (blockIdx.z = member * 6 + field), 256
-// threads.  Thread (b, m) = (tid / 16, tid % 16) loads the coefficients k = 16 m .. 16 m + 15
-// of pencil b (a contiguous 256- or 128-byte run), takes the pencil's largest magnitude
-// over its 16 lanes (shuffles, no LDS atomics) and writes the even and the odd eight of its
-// run, split and scaled, as 16-byte rows of the operand tiles (zeros past nz and for
-// missing pencils, so every tile entry a product reads is written).  Wave w then takes the
-// row tiles w, w + 4, .. of E and the same row tiles of O.  tab: the E fragments, then the
-// O fragments.
+// Persistent blocks of max(8, RTE) waves: wave w keeps the table fragments of row tile w
+// (E, and O when w < RTO) in registers for the whole launch, so the tables are read once
+// per wave.  Items (member, field, kx, 16 consecutive ky) go round-robin over the grid in
+// pairs: thread t < 512 loads for item 2 p + t / 256 the run of coefficients
+// k = 16 m .. 16 m + 15 of pencil b ((b, m) = ((t % 256) / 16, t % 16), contiguous), takes
+// the pencil's largest magnitude from its 16 lanes (shuffles) and writes the run, split
+// and scaled, as 16-byte rows of the operand tiles (zeros past nz and for missing pencils).
+// Operand tiles and scales are double buffered: the next pair's runs are in flight while
+// the current pair's products and stores run; one barrier per pair.
+// LDS (dynamic): [2 buffers][2 items][E hi | E lo | O hi | O lo][column][k / 2] halves.
 // -------------------------------------------------------------------------------------
-template <int FN, typename TI = double2>
-__global__ __launch_bounds__(256) void k_s2p_mfma(const TI* __restrict__ F, long fstride, long sF,
-                                                  float2* __restrict__ Y, long ystride, long sY, Dims d,
-                                                  const h8_t* __restrict__ tab, const int* __restrict__ ids) {
+template <int FN>
+struct S2pMfma {
+    using Sh = cheb_mfma::Shape<FN>;
+    static constexpr int waves = Sh::RTE > 8 ? Sh::RTE : 8;
+    static constexpr int threads = 64 * waves;
+    static constexpr int tile = 2 * cheb_mfma::kCols * (Sh::LDE + Sh::LDO);  // halves of one item's operands
+    static constexpr size_t lds = sizeof(_Float16) * 4 * tile;
+};
+
+template <int FN>
+__global__ __launch_bounds__(S2pMfma<FN>::threads) void k_s2p_mfma(const float2* __restrict__ F, long fstride,
+                                                                   long sF, float2* __restrict__ Y, long ystride,
+                                                                   long sY, Dims d, const h8_t* __restrict__ tab,
+                                                                   const int* __restrict__ ids, int nmem) {
     using namespace cheb_mfma;
     using Sh = Shape<FN>;
     constexpr int N = Sh::N, nz = Sh::nz, pz = Sh::pz;
+    constexpr int TILE = S2pMfma<FN>::tile;
     static_assert(nz <= 16 * 16 && 8 * 16 >= Sh::KSE * 32, "one 16-coefficient run per lane");
-    __shared__ __align__(16) _Float16 bE[2][kCols * Sh::LDE];  // [hi, lo][column][even k / 2]
-    __shared__ __align__(16) _Float16 bO[2][kCols * Sh::LDO];  // [hi, lo][column][(odd k - 1) / 2]
-    __shared__ int sexp[kPencils];
-    const int slot = ids[blockIdx.z / 6], field = blockIdx.z % 6;
-    F += slot * sF + field * fstride;
-    Y += slot * sY + field * ystride;
-    const int kx = blockIdx.y, ky0 = blockIdx.x * kPencils;
-    const int nb = min(kPencils, d.ny - ky0);
-    const int tid = threadIdx.x;
-    {
-        const int b = tid >> 4, m = tid & 15, k0 = 16 * m;
-        const bool have = b < nb;
-        TI a[16];
-        const TI* src = F + ((long)(ky0 + b) * d.hnx + kx) * nz + k0;
+    RPCF_LDS(_Float16);  // [2][2][TILE]
+    __shared__ int sexp[2][2][kPencils];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lc = lane & 15, lk = 8 * (lane >> 4);
+    const int tiles_y = (d.ny + kPencils - 1) / kPencils;
+    const int items = nmem * 6 * d.hnx * tiles_y, pairs = (items + 1) / 2;
+    // this wave's table fragments, for the whole launch
+    h8_t aeh[Sh::KSE], ael[Sh::KSE], aoh[Sh::KSO], aol[Sh::KSO];
+    const int rt = wave;
+    const bool has_e = rt < Sh::RTE, has_o = rt < Sh::RTO;
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            a[i].x = 0;
-            a[i].y = 0;
-            if (have && k0 + i < nz) a[i] = src[i];
-        }
+    for (int s = 0; s < Sh::KSE; s++) {
+        aeh[s] = has_e ? tab[((rt * Sh::KSE + s) * 2 + 0) * 64 + lane] : h8_t{};
+        ael[s] = has_e ? tab[((rt * Sh::KSE + s) * 2 + 1) * 64 + lane] : h8_t{};
+    }
+    const h8_t* tabO = tab + 2 * 64 * Sh::FE;
+#pragma unroll
+    for (int s = 0; s < Sh::KSO; s++) {
+        aoh[s] = has_o ? tabO[((rt * Sh::KSO + s) * 2 + 0) * 64 + lane] : h8_t{};
+        aol[s] = has_o ? tabO[((rt * Sh::KSO + s) * 2 + 1) * 64 + lane] : h8_t{};
+    }
+    // item -> (member slot, field, kx, ky0)
+    auto decode = [&](int it, int* slot, int* field, int* kx, int* ky0) {
+        const int t = it % tiles_y, r = it / tiles_y;
+        *kx = r % d.hnx;
+        const int mf = r / d.hnx;
+        *field = mf % 6;
+        *slot = ids[mf / 6];
+        *ky0 = t * kPencils;
+    };
+    const int lu = tid >> 8, lb = (tid >> 4) & 15, lm = tid & 15, k0 = 16 * lm;  // loader: item, pencil, run
+    float2 a[16];
+    auto load = [&](int pr) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) a[i] = make_float2(0.f, 0.f);
+        const int it = 2 * pr + lu;
+        if (tid >= 512 || pr >= pairs || it >= items) return;
+        int slot, field, kx, ky0;
+        decode(it, &slot, &field, &kx, &ky0);
+        if (ky0 + lb >= d.ny) return;
+        const float2* src = F + slot * sF + field * fstride + ((long)(ky0 + lb) * d.hnx + kx) * nz + k0;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (k0 + i < nz) a[i] = src[i];
+    };
+    auto split = [&](int pb) {
+        if (tid >= 512) return;
         float mx = 0.f;
 #pragma unroll
-        for (int i = 0; i < 16; i++) mx = fmaxf(mx, (float)fmax(fabs((double)a[i].x), fabs((double)a[i].y)));
+        for (int i = 0; i < 16; i++) mx = fmaxf(mx, fmaxf(fabsf(a[i].x), fabsf(a[i].y)));
         mx = fmaxf(mx, __shfl_xor(mx, 1));
         mx = fmaxf(mx, __shfl_xor(mx, 2));
         mx = fmaxf(mx, __shfl_xor(mx, 4));
         mx = fmaxf(mx, __shfl_xor(mx, 8));
         const int e = scale_exp(mx);
-        if (m == 0) sexp[b] = e;
-        // runs j = 8 m .. 8 m + 7 of the even (k = 2 j) and odd (k = 2 j + 1) coefficients
+        if (lm == 0) sexp[pb][lu][lb] = e;
         h8_t erh, erl, eih, eil, orh, orl, oih, oil;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const double xr = ldexp((double)a[2 * i].x, e), xi = ldexp((double)a[2 * i].y, e);
-            const double yr = ldexp((double)a[2 * i + 1].x, e), yi = ldexp((double)a[2 * i + 1].y, e);
-            erh[i] = (_Float16)(float)xr;
-            erl[i] = (_Float16)(float)(xr - (double)(float)erh[i]);
-            eih[i] = (_Float16)(float)xi;
-            eil[i] = (_Float16)(float)(xi - (double)(float)eih[i]);
-            orh[i] = (_Float16)(float)yr;
-            orl[i] = (_Float16)(float)(yr - (double)(float)orh[i]);
-            oih[i] = (_Float16)(float)yi;
-            oil[i] = (_Float16)(float)(yi - (double)(float)oih[i]);
+            const float xr = ldexpf(a[2 * i].x, e), xi = ldexpf(a[2 * i].y, e);
+            const float yr = ldexpf(a[2 * i + 1].x, e), yi = ldexpf(a[2 * i + 1].y, e);
+            erh[i] = (_Float16)xr;
+            erl[i] = (_Float16)(xr - (float)erh[i]);
+            eih[i] = (_Float16)xi;
+            eil[i] = (_Float16)(xi - (float)eih[i]);
+            orh[i] = (_Float16)yr;
+            orl[i] = (_Float16)(yr - (float)orh[i]);
+            oih[i] = (_Float16)yi;
+            oil[i] = (_Float16)(yi - (float)oih[i]);
         }
-        const int j0 = 8 * m;
+        _Float16* eh = lds + (pb * 2 + lu) * TILE;
+        _Float16* el = eh + kCols * Sh::LDE;
+        _Float16* oh = el + kCols * Sh::LDE;
+        _Float16* ol = oh + kCols * Sh::LDO;
+        const int j0 = 8 * lm;
         if (j0 < Sh::KSE * 32) {
-            *(h8_t*)&bE[0][b * Sh::LDE + j0] = erh;
-            *(h8_t*)&bE[1][b * Sh::LDE + j0] = erl;
-            *(h8_t*)&bE[0][(kPencils + b) * Sh::LDE + j0] = eih;
-            *(h8_t*)&bE[1][(kPencils + b) * Sh::LDE + j0] = eil;
+            *(h8_t*)&eh[lb * Sh::LDE + j0] = erh;
+            *(h8_t*)&el[lb * Sh::LDE + j0] = erl;
+            *(h8_t*)&eh[(kPencils + lb) * Sh::LDE + j0] = eih;
+            *(h8_t*)&el[(kPencils + lb) * Sh::LDE + j0] = eil;
         }
         if (j0 < Sh::KSO * 32) {
-            *(h8_t*)&bO[0][b * Sh::LDO + j0] = orh;
-            *(h8_t*)&bO[1][b * Sh::LDO + j0] = orl;
-            *(h8_t*)&bO[0][(kPencils + b) * Sh::LDO + j0] = oih;
-            *(h8_t*)&bO[1][(kPencils + b) * Sh::LDO + j0] = oil;
+            *(h8_t*)&oh[lb * Sh::LDO + j0] = orh;
+            *(h8_t*)&ol[lb * Sh::LDO + j0] = orl;
+            *(h8_t*)&oh[(kPencils + lb) * Sh::LDO + j0] = oih;
+            *(h8_t*)&ol[(kPencils + lb) * Sh::LDO + j0] = oil;
         }
-    }
+    };
+    int pr = blockIdx.x, pb = 0;
+    load(pr);
+    split(pb);
     __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6, lc = lane & 15, lk = 8 * (lane >> 4);
-    const h8_t* tabE = tab;
-    const h8_t* tabO = tab + 2 * 64 * Sh::FE;
-    const int b = lc;
-    const float sc = b < nb ? ldexpf(1.f, -sexp[b] - kTableScale) : 0.f;
-    for (int rt = wave; rt < Sh::RTE; rt += 4) {
-        f4_t eR = {0, 0, 0, 0}, eI = eR, oR = eR, oI = eR;
+    for (; pr < pairs; pr += gridDim.x, pb ^= 1) {
+        load(pr + gridDim.x);  // the next pair's runs, in flight during this pair's products
+        if (has_e) {
+#pragma unroll 1
+            for (int u = 0; u < 2; u++) {
+                const int it = 2 * pr + u;
+                if (it >= items) break;
+                int slot, field, kx, ky0;
+                decode(it, &slot, &field, &kx, &ky0);
+                const int nb = min(kPencils, d.ny - ky0);
+                const _Float16* eh = lds + (pb * 2 + u) * TILE;
+                const _Float16* el = eh + kCols * Sh::LDE;
+                const _Float16* oh = el + kCols * Sh::LDE;
+                const _Float16* ol = oh + kCols * Sh::LDO;
+                f4_t eR = {0, 0, 0, 0}, eI = eR, oR = eR, oI = eR;
 #pragma unroll
-        for (int s = 0; s < Sh::KSE; s++) {
-            const h8_t ah = tabE[((rt * Sh::KSE + s) * 2 + 0) * 64 + lane];
-            const h8_t al = tabE[((rt * Sh::KSE + s) * 2 + 1) * 64 + lane];
-            const int o = s * 32 + lk;
-            const h8_t brh = *(const h8_t*)&bE[0][lc * Sh::LDE + o], brl = *(const h8_t*)&bE[1][lc * Sh::LDE + o];
-            const h8_t bih = *(const h8_t*)&bE[0][(16 + lc) * Sh::LDE + o];
-            const h8_t bil = *(const h8_t*)&bE[1][(16 + lc) * Sh::LDE + o];
-            eR = mma3(ah, al, brh, brl, eR);
-            eI = mma3(ah, al, bih, bil, eI);
-        }
-        if (rt < Sh::RTO) {
+                for (int s = 0; s < Sh::KSE; s++) {
+                    const int o = s * 32 + lk;
+                    eR = mma3(aeh[s], ael[s], *(const h8_t*)&eh[lc * Sh::LDE + o], *(const h8_t*)&el[lc * Sh::LDE + o],
+                              eR);
+                    eI = mma3(aeh[s], ael[s], *(const h8_t*)&eh[(16 + lc) * Sh::LDE + o],
+                              *(const h8_t*)&el[(16 + lc) * Sh::LDE + o], eI);
+                }
+                if (has_o) {
 #pragma unroll
-            for (int s = 0; s < Sh::KSO; s++) {
-                const h8_t ah = tabO[((rt * Sh::KSO + s) * 2 + 0) * 64 + lane];
-                const h8_t al = tabO[((rt * Sh::KSO + s) * 2 + 1) * 64 + lane];
-                const int o = s * 32 + lk;
-                const h8_t brh = *(const h8_t*)&bO[0][lc * Sh::LDO + o], brl = *(const h8_t*)&bO[1][lc * Sh::LDO + o];
-                const h8_t bih = *(const h8_t*)&bO[0][(16 + lc) * Sh::LDO + o];
-                const h8_t bil = *(const h8_t*)&bO[1][(16 + lc) * Sh::LDO + o];
-                oR = mma3(ah, al, brh, brl, oR);
-                oI = mma3(ah, al, bih, bil, oI);
-            }
-        }
-        if (b < nb) {
-            float2* out = Y + (long)kx * pz * d.ny + ky0 + b;
+                    for (int s = 0; s < Sh::KSO; s++) {
+                        const int o = s * 32 + lk;
+                        oR = mma3(aoh[s], aol[s], *(const h8_t*)&oh[lc * Sh::LDO + o],
+                                  *(const h8_t*)&ol[lc * Sh::LDO + o], oR);
+                        oI = mma3(aoh[s], aol[s], *(const h8_t*)&oh[(16 + lc) * Sh::LDO + o],
+                                  *(const h8_t*)&ol[(16 + lc) * Sh::LDO + o], oI);
+                    }
+                }
+                const int b = lc;
+                if (b < nb) {
+                    const float sc = ldexpf(1.f, -sexp[pb][u][b] - kTableScale);
+                    float2* out = Y + slot * sY + field * ystride + (long)kx * pz * d.ny + ky0 + b;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int q = rt * 16 + (lane >> 4) * 4 + r;
-                if (q < N) {
-                    out[(long)q * d.ny] = make_float2((eR[r] + oR[r]) * sc, (eI[r] + oI[r]) * sc);
-                    out[(long)(FN - q) * d.ny] = make_float2((eR[r] - oR[r]) * sc, (eI[r] - oI[r]) * sc);
-                } else if (q == N) {
-                    out[(long)q * d.ny] = make_float2(eR[r] * sc, eI[r] * sc);
+                    for (int r = 0; r < 4; r++) {
+                        const int q = rt * 16 + (lane >> 4) * 4 + r;
+                        if (q < N) {
+                            out[(long)q * d.ny] = make_float2((eR[r] + oR[r]) * sc, (eI[r] + oI[r]) * sc);
+                            out[(long)(FN - q) * d.ny] = make_float2((eR[r] - oR[r]) * sc, (eI[r] - oI[r]) * sc);
+                        } else if (q == N) {
+                            out[(long)q * d.ny] = make_float2(eR[r] * sc, eI[r] * sc);
+                        }
+                    }
                 }
             }
         }
+        split(pb ^ 1);  // the other buffers: their last readers finished before the previous barrier
+        __syncthreads();
     }
 }
 
@@ -237,18 +292,19 @@ __global__ __launch_bounds__(256) void k_s2p_mfma(const TI* __restrict__ F, long
 // Block: 16 pencils (kx fixed) of one member (blockIdx.z); two waves per row tile of the
 // p <= N/2 rows (128 * RTS threads), wave 2 rt + c holding the real (c = 0) or imaginary
 // (c = 1) parts of the row tile rt of every product: after the per-pencil combinations
-// the products act on real and imaginary parts alike.  Two stages: h3 and h2 (from f_x,
-// f_y) into two operand sets, then h1 (from f_z, loaded during the first products).  Each
-// stage: fold and combine in FP32, one power-of-two scale per pencil and input (largest
-// magnitude through LDS), split into LDS, products.  The results go through LDS to
-// coalesced rows of N_v, N_eta (FP64) in the slot's current history buffers.
+// the products act on real and imaginary parts alike.  One stage: the three components
+// are loaded together, folded and combined in FP32 into h3, h2, h1, each with one
+// power-of-two scale per pencil (largest magnitude by shuffles and one LDS atomic per wave
+// and pencil), split into three operand sets in LDS; every Q0 fragment read from the table
+// then serves both h3 (N_eta) and h1 (N_v), the Q1 fragments h2.  The results go through
+// LDS to contiguous rows of N_v, N_eta (FP64) in the slot's current history buffers.
 // tab: fragments of Q0s, Q0a, Q1s, Q1a (in this order); s0, s1: scales of the M0 and M1 tables.
-// LDS (dynamic): two operand sets [set][hi, lo][32 columns][LDP (G+) | LDM (G-)] halves.
+// LDS (dynamic): three operand sets [set][hi, lo][32 columns][LDP (G+) | LDM (G-)] halves.
 // -------------------------------------------------------------------------------------
 template <int FN>
 constexpr size_t p2s_mfma_lds() {
     using Sh = cheb_mfma::Shape<FN>;
-    return sizeof(_Float16) * 2 * 2 * cheb_mfma::kCols * (Sh::LDP + Sh::LDM);
+    return sizeof(_Float16) * 3 * 2 * cheb_mfma::kCols * (Sh::LDP + Sh::LDM);
 }
 
 template <int FN>
@@ -282,23 +338,8 @@ __global__ __launch_bounds__(128 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
     auto gp_lo = [&](int st) { return lds + st * SET + kCols * Sh::LDP; };
     auto gm_hi = [&](int st) { return lds + st * SET + 2 * kCols * Sh::LDP; };
     auto gm_lo = [&](int st) { return lds + st * SET + 2 * kCols * Sh::LDP + kCols * Sh::LDM; };
-    {
-        // zero both sets once: padding k and ragged columns stay zero, the rest is overwritten
-        uint4* z = (uint4*)lds;
-        for (int i = tid; i < 2 * SET * 2 / 16; i += NT) z[i] = make_uint4(0, 0, 0, 0);
-        if (tid < 3 * kPencils) (&smax[0][0])[tid] = 0u;
-    }
-    // pencil scalars of pencil b: alpha, beta, k^2; true for pencil (0,0)
-    auto scal = [&](int b, float* ia, float* ib, float* k2) {
-        const int ky = ky0 + b;
-        const double a_ = (double)kx / aphi;
-        const double b_ = ky >= d.ny / 2 + 1 ? (double)(ky - d.ny) / beta : (double)ky / beta;
-        *ia = (float)a_;
-        *ib = (float)b_;
-        *k2 = (float)(a_ * a_ + b_ * b_);
-        return has_mean && b == 0;
-    };
     // samples q and 2N - q of component g for every item (q = i / nb, b = i % nb)
+    float2 xa[KQ], xb[KQ], ya[KQ], yb[KQ], za[KQ], zb[KQ];
     auto load = [&](int g, float2* fa, float2* fb) {
         const float2* in = Z1 + g * zstride + (long)kx * pz * d.ny + ky0;
 #pragma unroll
@@ -311,6 +352,15 @@ __global__ __launch_bounds__(128 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
             }
         }
     };
+    load(0, xa, xb);
+    load(1, ya, yb);
+    load(2, za, zb);
+    {
+        // zero the three sets: padding k and ragged columns stay zero, the rest is overwritten
+        uint4* z = (uint4*)lds;
+        for (int i = tid; i < 3 * SET * 2 / 16; i += NT) z[i] = make_uint4(0, 0, 0, 0);
+        if (tid < 3 * kPencils) (&smax[0][0])[tid] = 0u;
+    }
     auto fold = [&](float2 a, float2 b, int q, float2* gp, float2* gm) {
         *gp = q < N ? make_float2(a.x + b.x, a.y + b.y) : a;
         *gm = q < N ? make_float2(a.x - b.x, a.y - b.y) : make_float2(0.f, 0.f);
@@ -335,12 +385,12 @@ __global__ __launch_bounds__(128 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
             }
         }
     };
-    auto split_set = [&](int st, int h, const float2* hp, const float2* hm) {
+    auto split_set = [&](int st, const float2* hp, const float2* hm) {
 #pragma unroll
         for (int u = 0; u < KQ; u++) {
             const int i = tid + u * NT;
             if (i < total) {
-                const int q = i / nb, b = i - q * nb, e = scale_exp(__uint_as_float(smax[h][b]));
+                const int q = i / nb, b = i - q * nb, e = scale_exp(__uint_as_float(smax[st][b]));
                 split_store(&gp_hi(st)[b * Sh::LDP + q], &gp_lo(st)[b * Sh::LDP + q], ldexpf(hp[u].x, e));
                 split_store(&gp_hi(st)[(kPencils + b) * Sh::LDP + q], &gp_lo(st)[(kPencils + b) * Sh::LDP + q],
                             ldexpf(hp[u].y, e));
@@ -352,107 +402,95 @@ __global__ __launch_bounds__(128 * cheb_mfma::Shape<FN>::RTS) void k_p2s_mfma(
             }
         }
     };
-    // acc += table (row tile rt, ks k-steps) x operand columns of this wave's part ct
-    auto prod = [&](const h8_t* t, int ks, const _Float16* bh, const _Float16* bl, int ld, f4_t acc) {
-        const int col = ct * kPencils + lc;
+    // h3 = i alpha f_y - i beta f_x -> set 0; h2 = i alpha f_x + i beta f_y -> set 1;
+    // h1 = k^2 f_z -> set 2.  Pencil (0,0): h3 = f_y, h2 = 0, h1 = f_x.
+    float2 h3p[KQ], h3m[KQ], h2p[KQ], h2m[KQ], h1p[KQ], h1m[KQ];
+    {
+        float m3[KQ], m2[KQ], m1[KQ];
 #pragma unroll
-        for (int s = 0; s < ks; s++) {
-            const h8_t ah = t[((rt * ks + s) * 2 + 0) * 64 + lane];
-            const h8_t al = t[((rt * ks + s) * 2 + 1) * 64 + lane];
-            const int o = col * ld + s * 32 + lk;
-            acc = mma3(ah, al, *(const h8_t*)&bh[o], *(const h8_t*)&bl[o], acc);
+        for (int u = 0; u < KQ; u++) {
+            m3[u] = m2[u] = m1[u] = 0.f;
+            const int i = tid + u * NT;
+            if (i < total) {
+                const int q = i / nb, b = i - q * nb, ky = ky0 + b;
+                const double a_ = (double)kx / aphi;
+                const double b_ = ky >= d.ny / 2 + 1 ? (double)(ky - d.ny) / beta : (double)ky / beta;
+                const float ia = (float)a_, ib = (float)b_, k2 = (float)(a_ * a_ + b_ * b_);
+                const bool mean = has_mean && b == 0;
+                float2 xp, xm, yp, ym, zp, zm;
+                fold(xa[u], xb[u], q, &xp, &xm);
+                fold(ya[u], yb[u], q, &yp, &ym);
+                fold(za[u], zb[u], q, &zp, &zm);
+                // i a y - i b x = (-a y.im + b x.im, a y.re - b x.re)
+                h3p[u] = mean ? yp : make_float2(-ia * yp.y + ib * xp.y, ia * yp.x - ib * xp.x);
+                h3m[u] = mean ? ym : make_float2(-ia * ym.y + ib * xm.y, ia * ym.x - ib * xm.x);
+                // i a x + i b y = (-a x.im - b y.im, a x.re + b y.re)
+                h2p[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xp.y - ib * yp.y, ia * xp.x + ib * yp.x);
+                h2m[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xm.y - ib * ym.y, ia * xm.x + ib * ym.x);
+                h1p[u] = mean ? xp : make_float2(k2 * zp.x, k2 * zp.y);
+                h1m[u] = mean ? xm : make_float2(k2 * zm.x, k2 * zm.y);
+                m3[u] = mag(h3p[u], h3m[u]);
+                m2[u] = mag(h2p[u], h2m[u]);
+                m1[u] = mag(h1p[u], h1m[u]);
+            }
         }
-        return acc;
-    };
+        __syncthreads();  // sets zeroed, maxima cleared
+        pencil_max(0, m3);
+        pencil_max(1, m2);
+        pencil_max(2, m1);
+    }
+    __syncthreads();
+    split_set(0, h3p, h3m);
+    split_set(1, h2p, h2m);
+    split_set(2, h1p, h1m);
+    __syncthreads();
+    // N_eta: sym rows Q0s G+, anti rows Q0a G- of h3 (set 0); N_v: the same tables on h1 (set 2)
+    // and sym rows Q1s G-, anti rows Q1a G+ of h2 (set 1)
     const h8_t* t0s = tab;
     const h8_t* t0a = t0s + 2 * 64 * Sh::F0S;
     const h8_t* t1s = t0a + 2 * 64 * Sh::F0A;
     const h8_t* t1a = t1s + 2 * 64 * Sh::F1S;
+    const int col = ct * kPencils + lc;
     const f4_t zero4 = {0, 0, 0, 0};
     f4_t es = zero4, ea = zero4, v1s = zero4, v1a = zero4, v0s = zero4, v0a = zero4;
-    float2 za[KQ], zb[KQ];
-    __syncthreads();  // tiles zeroed, maxima cleared
-    {
-        // stage 1: h3 = i alpha f_y - i beta f_x (pencil (0,0): f_y) -> set 0;
-        //          h2 = i alpha f_x + i beta f_y (pencil (0,0): 0)   -> set 1
-        float2 h3p[KQ], h3m[KQ], h2p[KQ], h2m[KQ];
-        {
-            float2 xa[KQ], xb[KQ], ya[KQ], yb[KQ];
-            load(0, xa, xb);
-            load(1, ya, yb);
-            float m3[KQ], m2[KQ];
 #pragma unroll
-            for (int u = 0; u < KQ; u++) {
-                m3[u] = m2[u] = 0.f;
-                const int i = tid + u * NT;
-                if (i < total) {
-                    const int q = i / nb, b = i - q * nb;
-                    float ia, ib, k2;
-                    const bool mean = scal(b, &ia, &ib, &k2);
-                    float2 xp, xm, yp, ym;
-                    fold(xa[u], xb[u], q, &xp, &xm);
-                    fold(ya[u], yb[u], q, &yp, &ym);
-                    // i a y - i b x = (-a y.im + b x.im, a y.re - b x.re)
-                    h3p[u] = mean ? yp : make_float2(-ia * yp.y + ib * xp.y, ia * yp.x - ib * xp.x);
-                    h3m[u] = mean ? ym : make_float2(-ia * ym.y + ib * xm.y, ia * ym.x - ib * xm.x);
-                    // i a x + i b y = (-a x.im - b y.im, a x.re + b y.re)
-                    h2p[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xp.y - ib * yp.y, ia * xp.x + ib * yp.x);
-                    h2m[u] = mean ? make_float2(0.f, 0.f) : make_float2(-ia * xm.y - ib * ym.y, ia * xm.x + ib * ym.x);
-                    m3[u] = mag(h3p[u], h3m[u]);
-                    m2[u] = mag(h2p[u], h2m[u]);
-                }
-            }
-            pencil_max(0, m3);
-            pencil_max(1, m2);
-        }
-        __syncthreads();
-        split_set(0, 0, h3p, h3m);
-        split_set(1, 1, h2p, h2m);
+    for (int s = 0; s < Sh::KSP; s++) {
+        const h8_t ah = t0s[((rt * Sh::KSP + s) * 2 + 0) * 64 + lane];
+        const h8_t al = t0s[((rt * Sh::KSP + s) * 2 + 1) * 64 + lane];
+        const int o = col * Sh::LDP + s * 32 + lk;
+        es = mma3(ah, al, *(const h8_t*)&gp_hi(0)[o], *(const h8_t*)&gp_lo(0)[o], es);
+        v0s = mma3(ah, al, *(const h8_t*)&gp_hi(2)[o], *(const h8_t*)&gp_lo(2)[o], v0s);
     }
-    load(2, za, zb);  // f_z for h1, in flight during the first products
-    __syncthreads();
-    // N_eta: sym rows Q0s G+, anti rows Q0a G- of h3 (set 0); N_v (M1 part): sym rows Q1s G-,
-    // anti rows Q1a G+ of h2 (set 1)
-    es = prod(t0s, Sh::KSP, gp_hi(0), gp_lo(0), Sh::LDP, es);
-    v1s = prod(t1s, Sh::KSM, gm_hi(1), gm_lo(1), Sh::LDM, v1s);
+#pragma unroll
+    for (int s = 0; s < Sh::KSM; s++) {
+        const h8_t ah = t1s[((rt * Sh::KSM + s) * 2 + 0) * 64 + lane];
+        const h8_t al = t1s[((rt * Sh::KSM + s) * 2 + 1) * 64 + lane];
+        const int o = col * Sh::LDM + s * 32 + lk;
+        v1s = mma3(ah, al, *(const h8_t*)&gm_hi(1)[o], *(const h8_t*)&gm_lo(1)[o], v1s);
+    }
     if (anti) {
-        ea = prod(t0a, Sh::KSM, gm_hi(0), gm_lo(0), Sh::LDM, ea);
-        v1a = prod(t1a, Sh::KSP, gp_hi(1), gp_lo(1), Sh::LDP, v1a);
-    }
-    {
-        // stage 2: h1 = k^2 f_z (pencil (0,0): f_x) -> set 0
-        float2 h1p[KQ], h1m[KQ];
-        float m1[KQ];
-        const float2* x0 = Z1 + (long)kx * pz * d.ny + ky0;  // f_x of pencil (0,0)
 #pragma unroll
-        for (int u = 0; u < KQ; u++) {
-            m1[u] = 0.f;
-            const int i = tid + u * NT;
-            if (i < total) {
-                const int q = i / nb, b = i - q * nb;
-                float ia, ib, k2;
-                const bool mean = scal(b, &ia, &ib, &k2);
-                float2 zp, zm;
-                if (mean) fold(x0[(long)q * d.ny], x0[(long)(FN - q) * d.ny], q, &zp, &zm);
-                else fold(za[u], zb[u], q, &zp, &zm);
-                h1p[u] = mean ? zp : make_float2(k2 * zp.x, k2 * zp.y);
-                h1m[u] = mean ? zm : make_float2(k2 * zm.x, k2 * zm.y);
-                m1[u] = mag(h1p[u], h1m[u]);
-            }
+        for (int s = 0; s < Sh::KSM; s++) {
+            const h8_t ah = t0a[((rt * Sh::KSM + s) * 2 + 0) * 64 + lane];
+            const h8_t al = t0a[((rt * Sh::KSM + s) * 2 + 1) * 64 + lane];
+            const int o = col * Sh::LDM + s * 32 + lk;
+            ea = mma3(ah, al, *(const h8_t*)&gm_hi(0)[o], *(const h8_t*)&gm_lo(0)[o], ea);
+            v0a = mma3(ah, al, *(const h8_t*)&gm_hi(2)[o], *(const h8_t*)&gm_lo(2)[o], v0a);
         }
-        pencil_max(2, m1);
-        __syncthreads();  // also: set 0 is no longer read by the stage-1 products
-        split_set(0, 2, h1p, h1m);
+#pragma unroll
+        for (int s = 0; s < Sh::KSP; s++) {
+            const h8_t ah = t1a[((rt * Sh::KSP + s) * 2 + 0) * 64 + lane];
+            const h8_t al = t1a[((rt * Sh::KSP + s) * 2 + 1) * 64 + lane];
+            const int o = col * Sh::LDP + s * 32 + lk;
+            v1a = mma3(ah, al, *(const h8_t*)&gp_hi(1)[o], *(const h8_t*)&gp_lo(1)[o], v1a);
+        }
     }
-    __syncthreads();
-    v0s = prod(t0s, Sh::KSP, gp_hi(0), gp_lo(0), Sh::LDP, v0s);
-    if (anti) v0a = prod(t0a, Sh::KSM, gm_hi(0), gm_lo(0), Sh::LDM, v0a);
     // combine: lane holds part ct of pencil b = lc, rows 16 rt + 4 (lane / 16) + r; results go
     // to LDS as [N_v, N_eta][pencil][row] complex doubles, then out in rows of nz
     const int b = lc;
-    const float e3 = (float)__uint_as_float(smax[0][b]), e2 = __uint_as_float(smax[1][b]), e1 = __uint_as_float(smax[2][b]);
-    const double se = ldexp(1.0, -scale_exp(e3) - s0), sv1 = ldexp(1.0, -scale_exp(e2) - s1),
-                 sv0 = ldexp(1.0, -scale_exp(e1) - s0);
+    const double se = ldexp(1.0, -scale_exp(__uint_as_float(smax[0][b])) - s0);
+    const double sv1 = ldexp(1.0, -scale_exp(__uint_as_float(smax[1][b])) - s1);
+    const double sv0 = ldexp(1.0, -scale_exp(__uint_as_float(smax[2][b])) - s0);
     __syncthreads();  // operand sets no longer read: the output staging reuses them
     double* outv = (double*)lds;                        // [pencil][nz][2]
     double* oute = outv + 2 * kPencils * nz;

@@ -460,6 +460,95 @@ __global__ __launch_bounds__(256, tv_waves(S)) void k_lin_tv_mfma(const double2*
     }
 }
 
+// The same products from persistent blocks: block (i, 5 j + m) copies the table of operator
+// m of member j (both parities, all RT row tiles: 2 RT S 64 doubles) into LDS once, then its
+// four waves take the groups of 8 pencils g = 4 i + w, 4 i + w + 4 G, .. (G blocks per
+// operator), the next group's even and odd coefficients loaded while the current group's
+// RT x 2 chains of S products run (one block, one wave per SIMD per compute unit).  The table is read from L2 once per block instead of once
+// per 32 pencils.  Same tiles and summation order as k_lin_tv_mfma: bit-identical results.
+template <int S>
+__global__ __launch_bounds__(256) void k_lin_tv_stream(const double2* __restrict__ sv, const double2* __restrict__ se,
+                                                       long sS, double2* __restrict__ W, long sWl, Dims d,
+                                                       const double* __restrict__ Tf, long sTf, int RT,
+                                                       const int* __restrict__ ids) {
+    extern __shared__ __align__(16) double tvT[];  // [2][RT][S][64]
+    const int m = blockIdx.y % 5, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        const long slot = ids[blockIdx.y / 5];
+        sv += slot * sS;
+        se += slot * sS;
+        W += slot * sWl;
+        Tf += slot * sTf;
+    }
+    const int cnt = RT * S * 64;  // double2 elements of the operator's table (both parities)
+    {
+        const double2* src = (const double2*)(Tf + (long)m * 2 * RT * S * 64);
+        for (int i = tid; i < cnt; i += blockDim.x) ((double2*)tvT)[i] = src[i];
+    }
+    __syncthreads();
+    const int n = d.nz, N = d.N, col = lane & 15, kq = lane >> 4, rows = N / 2 + 1;
+    const int ngroups = (d.P + 7) / 8, stride = gridDim.x * 4;
+    const double2* x = m < 3 ? sv : se;
+    double* Wm0 = (double*)(W + (long)m * d.P * n) + (col & 1);
+    double be[S], bo[S];
+    auto load = [&](int g) {
+        const int p = g * 8 + (col >> 1);
+        const bool okp = g < ngroups && p < d.P;
+        const double* xp = (const double*)(x + (long)(okp ? p : 0) * n) + (col & 1);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int je = 2 * (4 * s + kq), jo = je + 1;
+            be[s] = xp[2 * min(je, n - 1)] * ((okp && je < n) ? 1.0 : 0.0);
+            bo[s] = xp[2 * min(jo, n - 1)] * ((okp && jo < n) ? 1.0 : 0.0);
+        }
+    };
+    int g = blockIdx.x * 4 + wave;
+    load(g);
+    for (; g < ngroups; g += stride) {
+        double ce[S], co[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) ce[s] = be[s], co[s] = bo[s];
+        load(g + stride);  // the next group's coefficients, in flight during this group's products
+        const int p = g * 8 + (col >> 1);
+        double* Wm = Wm0 + (long)p * 2 * n;
+        // a last row tile that holds only the centre row p = N/2 (N/2 a multiple of 16) is a
+        // dot product on the vector units: lane (column, kq) sums its k-steps' row-0 entries
+        // times its coefficients, then the four kq lanes of the column add up
+        const int RTm = 16 * (RT - 1) + 1 == rows ? RT - 1 : RT;
+        if (RTm < RT) {
+            double e = 0.0, o = 0.0;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                e = fma(tvT[(RTm * S + s) * 64 + 16 * kq], ce[s], e);
+                o = fma(tvT[((RT + RTm) * S + s) * 64 + 16 * kq], co[s], o);
+            }
+            e += __shfl_xor(e, 16);
+            o += __shfl_xor(o, 16);
+            e += __shfl_xor(e, 32);
+            o += __shfl_xor(o, 32);
+            if (kq == 0 && p < d.P) Wm[2 * (N / 2)] = e + o;
+        }
+        for (int t = 0; t < RTm; t++) {
+            rpcf_v4d ae = {0.0, 0.0, 0.0, 0.0}, ao = ae;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                ae = __builtin_amdgcn_mfma_f64_16x16x4f64(tvT[(t * S + s) * 64 + lane], ce[s], ae, 0, 0, 0);
+                ao = __builtin_amdgcn_mfma_f64_16x16x4f64(tvT[((RT + t) * S + s) * 64 + lane], co[s], ao, 0, 0, 0);
+            }
+            if (p < d.P) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = 16 * t + kq + 4 * r;
+                    if (row < rows) {
+                        Wm[2 * row] = ae[r] + ao[r];
+                        if (N - row != row) Wm[2 * (N - row)] = ae[r] - ao[r];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // y_q = M x_q for the two vectors of a slot with every thread of the block: G = nthr / n
 // groups of n threads (thread t = g*n + r owns row r), group g takes the columns
 // j = g, g + G, g + 2G, .. (fused multiply-adds, j ascending), so that the block reads the

@@ -457,14 +457,16 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
             [&] {
                 if constexpr (!f64) {
                     if (c->use_mfma) {
-                        dim3 gm((d.ny + cheb_mfma::kPencils - 1) / cheb_mfma::kPencils, d.hnx, 6 * n);
+                        // persistent blocks: one per compute unit, pairs of items round-robin
+                        const int items = n * 6 * d.hnx * ((d.ny + cheb_mfma::kPencils - 1) / cheb_mfma::kPencils);
+                        const int grid = std::min((items + 1) / 2, c->ncu);
                         switch (2 * d.N) {
 #define RPCF_CASE(L)                                                                                               \
     case L:                                                                                                        \
         if constexpr (L <= 384)                                                                                    \
-            k_s2p_mfma<(L <= 384 ? L : 384), float2><<<gm, 256, 0, c->stream>>>((const float2*)c->F, fS, c->sF, Y,  \
-                                                                               yS, c->sY, d,                       \
-                                                                       (const h8_t*)c->MT, ids);                  \
+            k_s2p_mfma<(L <= 384 ? L : 384)><<<grid, S2pMfma<(L <= 384 ? L : 384)>::threads,                     \
+                                               S2pMfma<(L <= 384 ? L : 384)>::lds, c->stream>>>(                  \
+                (const float2*)c->F, fS, c->sF, Y, yS, c->sY, d, (const h8_t*)c->MT, ids, n);                     \
         break;
                             RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
@@ -627,11 +629,21 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
         [&] {
             const size_t lds = sizeof(double) * 2 * kTvChunk * c->tv_s * 64;
             dim3 g((d.P + kTvBlockPencils - 1) / kTvBlockPencils, 5 * n);
+            // persistent blocks with an operator's whole table in LDS when it fits (2N <= 256):
+            // about one block per compute unit (two when the table is small)
+            const size_t lds_stream = sizeof(double) * 2 * c->tv_rt * c->tv_s * 64;
+            const bool tv_stream = c->tv_s <= 17 && lds_stream <= (size_t)c->lds_limit;
+            const int per_cu = lds_stream * 2 <= (size_t)c->lds_limit ? 2 : 1;
+            const int gs = std::max(1, std::min((d.P + 31) / 32, per_cu * c->ncu / (5 * n)));
             switch (2 * d.N) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
-        k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, \
-                                                                   c->sTf, c->tv_rt, ids);                      \
+        if (tv_stream)                                                                                           \
+            k_lin_tv_stream<(L / 4 + 4) / 4><<<dim3(gs, 5 * n), 256, lds_stream, c->stream>>>(                    \
+                c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, c->sTf, c->tv_rt, ids);                            \
+        else                                                                                                     \
+            k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d,    \
+                                                                       c->TF, c->sTf, c->tv_rt, ids);           \
         break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
@@ -992,6 +1004,7 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_lamb_phys<L>, lim);   \
     allow_lds(k_p2s_nl<L>, lim);      \
     allow_lds(k_lin_tv_mfma<(L / 4 + 4) / 4>, lim);      \
+    allow_lds(k_lin_tv_stream<(L / 4 + 4) / 4>, lim);    \
     allow_lds(k_cheb_s2p<1, L, float>, lim);              \
     allow_lds(k_span_inv<L, float>, lim);                 \
     allow_lds(k_span_fwd<L, float>, lim);                 \
@@ -1003,7 +1016,8 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_p2s_nl2<L, float>, lim);                  \
     allow_lds(k_velocity_tile<L>, lim);                   \
     allow_lds(k_velocity_tile<L, float2>, lim);           \
-    allow_lds(k_p2s_mfma<(L <= 384 ? L : 384)>, lim);
+    allow_lds(k_p2s_mfma<(L <= 384 ? L : 384)>, lim);    \
+    allow_lds(k_s2p_mfma<(L <= 384 ? L : 384)>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     allow_lds(k_cheb_p2s, lim);
