@@ -47,32 +47,48 @@ def device_sync():
 
 
 def kernel_bytes(s, prm):
-    """algorithmic HBM bytes per launch of each kernel: the arrays it must read and
-    write once (complex FP64 = 16 B)."""
+    """algorithmic HBM bytes per launch of each kernel: the arrays it must read and write
+    once.  FP64 form: complex FP64 (16 B) throughout, solves with the full inverses.  Mixed
+    form (precision 1; channel profile, matrix-core Chebyshev transforms): the velocity and
+    vorticity fields, the padded intermediates and the Lamb vector in complex FP32 (8 B), the
+    state, nonlinear terms and operator outputs in FP64, and the solves with the two parity
+    blocks of each inverse (nz x (N/2 + 1) complex entries).  Precision 2: FP64 with the
+    parity blocks."""
     P, nz, hnx, ny, pz = s["P"], s["nz"], s["hnx"], s["ny"], s["pz"]
     my, nxh = prm.my, s["nx"] // 2
+    parity = prm.precision != 0 and prm.u_profile[1] == 0.0
+    mixed = prm.precision == 1 and 2 * (nz - 1) <= 384 and os.environ.get("RPCF_MIXED_MFMA", "1") != "0"
+    h = 0.5 if prm.precision == 1 else 1.0  # FP32 intermediates of the mixed form
     S = P * nz * 16                    # one spectral field
-    Sp = hnx * ny * pz * 16            # one field on the padded wall-normal points
-    Wb = pz * my * nxh * 16            # one field after the span transform (kx < nx/2)
-    Zb = pz * my * hnx * 16            # one Lamb component after the x transform
+    F = S * (0.5 if mixed else 1.0)    # one velocity / vorticity field (FP32 for the matrix-core form)
+    Sp = hnx * ny * pz * 16 * h        # one field on the padded wall-normal points
+    Wb = pz * my * nxh * 16 * h        # one field after the span transform (kx < nx/2)
+    Zb = pz * my * hnx * 16 * h        # one Lamb component after the x transform
     M1 = nz * nz * 16                  # one inverse matrix
+    if parity:
+        M1 = nz * ((nz - 1) // 2 + 1) * 16  # its parity blocks: nz rows x (N/2 + 1) pair columns
     nslot = hnx * (ny // 2 + 1)        # matrix slots: pencils +-ky share one
-    return {
-        "k_cheb_s2p": 6 * S + 6 * Sp,
+    out = {
+        "k_cheb_s2p": 6 * F + 6 * Sp,
         "k_span_inv": 6 * Sp + 6 * Wb,
         "k_lamb_phys": 6 * Wb + 3 * Zb,
         "k_span_fwd": 3 * Zb + 3 * Sp,
         "k_p2s_nl": 3 * Sp + 2 * S,
         "k_lin_tv": 2 * S + 5 * S,
         "k_lin_solve": 2 * nslot * M1 + 11 * S,
-        "k_velocity": 8 * S,
+        "k_velocity": 2 * S + 6 * F,
     }
+    return {k: int(v) for k, v in out.items()}
 
 
-def read_traffic(kernel):
+def read_traffic(kernel, precision=0):
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
-    committed in profiles/traffic_<kernel>_c1.txt (line 'traffic_bytes_per_launch: N')."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_c1.txt")
+    committed in profiles/traffic_<kernel>_c1.txt (FP64) or traffic_<kernel>_c1_mixed.txt
+    (line 'traffic_bytes_per_launch: N')."""
+    tag = {0: "", 1: "_mixed"}.get(precision)
+    if tag is None:
+        return None
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_c1{tag}.txt")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -167,12 +183,14 @@ def mixed_leg(prm64, v0, e0, warmup, steps, ms64, final64, precision=1, table_st
         sim.step(table_steps)
         table = {k: round(t / c, 5) for k, (t, c) in kernel_table(dll, sim.h).items()}
         assert dll.rpcf_timing_kernels(sim.h, 0) == 0
+        nbytes = kernel_bytes(rpcf.sizes(dll, prm), prm)
+    frac = {k: round(nbytes[k] / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k, t in table.items() if k in nbytes}
     err = {"v": float(np.abs(v - final64[0]).max() / np.abs(final64[0]).max()),
            "eta": float(np.abs(e - final64[1]).max() / np.abs(final64[1]).max())}
     log(f"precision {precision}: {ms:.4f} ms/step ({ms64 / ms:.2f}x FP64), error after {1 + warmup + steps} steps {err}")
     log(f"precision {precision} kernels (ms): {table}")
     return {"ms_per_step": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3), "steps_compared": 1 + warmup + steps,
-            "max_rel_error": err, "kernel_avg_ms": table,
+            "max_rel_error": err, "kernel_avg_ms": table, "kernel_hbm_frac": frac,
             "dtype": "f32 transforms (split-f16 MFMA Chebyshev) / f64 solves" if precision == 1
             else "f64, parity-block solves"}
 
@@ -240,6 +258,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
+                    help="form of the headline leg (0: FP64, the metric; 1: mixed; 2: FP64 with parity-block "
+                         "solves); with 1 or 2 the other legs are skipped (for profiling one form)")
     ap.add_argument("--ensemble", type=int, default=64, metavar="N",
                     help="members of the mixed-size ensemble leg (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
@@ -263,6 +284,9 @@ def main():
     os.environ.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
 
     prm = rpcf.make_params(**C1)
+    prm.precision = args.precision
+    if args.precision:
+        args.no_mixed, args.ensemble, args.serve, args.no_cpu = True, 0, 0.0, True
     t0 = time.perf_counter()
     sim = rpcf.Simulation(rpcf.HIP_LIB, prm)
     t_create = time.perf_counter() - t0
@@ -317,7 +341,7 @@ def main():
     _, dtot, dcnt = kernel_times()[dom]
     dgbs = nbytes[dom] / (dtot / dcnt * 1e-3) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(dgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dgbs / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom),
+                "frac": round(dgbs / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom, args.precision),
                 "avg_ms": round(dtot / dcnt, 5), "launches": dcnt}
     assert dll.rpcf_timing_kernels(sim.h, 0) == 0
     final64 = sim.get_state()
@@ -350,7 +374,7 @@ def main():
             f"of {cpu['cpu_model']}; GPU-vs-oracle parity after 1 step {parity:.2e}")
 
     out = {
-        "metric": "ns/gridpoint/step at 128x129x128 (FP64)",
+        "metric": "ns/gridpoint/step at 128x129x128 (%s)" % ("FP64", "mixed", "FP64, parity-block solves")[args.precision],
         "value": round(ms_step * 1e6 / npts, 4),
         "unit": "ns/gridpoint/step",
         "n_gpus": 1,
@@ -360,9 +384,10 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision != 1 else "f32 transforms (split-f16 MFMA Chebyshev) / f64 solves",
         "data": "synthetic (seeded wall-satisfying perturbation, seed 0)",
-        "config": {"workload": "channel 128x129x128 FP64, constant flow rate", "grid": "128x129x128",
+        "config": {"workload": "channel 128x129x128 %s, constant flow rate" % ("FP64", "mixed", "FP64 parity-block solves")[args.precision],
+                   "grid": "128x129x128",
                    "padded": "192x257x192", "Re": 3300, "dt": 0.002, "Lx": "4pi", "Lz": "4pi/3",
                    "parallelism": "single GPU"},
         "steps_per_s": round(1e3 / ms_step, 2),
