@@ -173,11 +173,14 @@ inline void s2p_tables(int N, std::vector<_Float16>& out) {
     mfma_fragments(O, N, no, 14, out);
 }
 
-// p2s + derivative + unpadded s2p of length 2N (see k_p2s_mfma): M0 = S P, M1 = S D P with
-// P the padded p2s (source/transform.cu:513-536, 538-658), D the Chebyshev derivative
-// (ddz_sm, source/RPCFKernels.cu:476-520), S the values at the nz collocation points;
-// then the folded tables Q0s, Q0a, Q1s, Q1a.  s0, s1: scales of the M0 and M1 tables.
-inline void p2s_tables(int N, std::vector<_Float16>& out, int* s0, int* s1) {
+// p2s + derivative + unpadded s2p of length 2N (k_p2s_mfma, k_p2s_dmfma): M0 = S P,
+// M1 = S D P with P the padded p2s (source/transform.cu:513-536, 538-658), D the Chebyshev
+// derivative (ddz_sm, source/RPCFKernels.cu:476-520), S the values at the nz collocation
+// points; folded by reflection parity into Q0s, Q0a, Q1s, Q1a (rows p <= N/2, p < N/2).
+struct P2sFolded {
+    std::vector<long double> q0s, q0a, q1s, q1a;  // [N/2+1][N+1], [N/2][N], [N/2+1][N], [N/2][N+1]
+};
+inline P2sFolded p2s_folded(int N) {
     const int nz = N + 1, pz = 2 * N + 1;
     std::vector<long double> P((size_t)nz * pz), DP((size_t)nz * pz, 0.0L), M0((size_t)nz * pz), M1((size_t)nz * pz);
     for (int k = 0; k < nz; k++)
@@ -200,27 +203,64 @@ inline void p2s_tables(int N, std::vector<_Float16>& out, int* s0, int* s1) {
             M1[(size_t)p * pz + q] = b;
         }
     const int rs = N / 2 + 1, ra = N / 2;
-    std::vector<long double> q0s((size_t)rs * (N + 1)), q0a((size_t)ra * N), q1s((size_t)rs * N), q1a((size_t)ra * (N + 1));
+    P2sFolded f;
+    f.q0s.resize((size_t)rs * (N + 1));
+    f.q0a.resize((size_t)ra * N);
+    f.q1s.resize((size_t)rs * N);
+    f.q1a.resize((size_t)ra * (N + 1));
     for (int p = 0; p < rs; p++)
         for (int q = 0; q <= N; q++) {
             const long double m0 = M0[(size_t)p * pz + q], m0r = M0[(size_t)p * pz + 2 * N - q];
             const long double m1 = M1[(size_t)p * pz + q], m1r = M1[(size_t)p * pz + 2 * N - q];
-            q0s[(size_t)p * (N + 1) + q] = q < N ? (m0 + m0r) / 2 : m0;
-            if (q < N) q1s[(size_t)p * N + q] = (m1 - m1r) / 2;
+            f.q0s[(size_t)p * (N + 1) + q] = q < N ? (m0 + m0r) / 2 : m0;
+            if (q < N) f.q1s[(size_t)p * N + q] = (m1 - m1r) / 2;
             if (p < ra) {
-                if (q < N) q0a[(size_t)p * N + q] = (m0 - m0r) / 2;
-                q1a[(size_t)p * (N + 1) + q] = q < N ? (m1 + m1r) / 2 : m1;
+                if (q < N) f.q0a[(size_t)p * N + q] = (m0 - m0r) / 2;
+                f.q1a[(size_t)p * (N + 1) + q] = q < N ? (m1 + m1r) / 2 : m1;
             }
         }
-    std::vector<long double> all0(q0s), all1(q1s);
-    all0.insert(all0.end(), q0a.begin(), q0a.end());
-    all1.insert(all1.end(), q1a.begin(), q1a.end());
+    return f;
+}
+
+// split-FP16 fragments of the folded tables (k_p2s_mfma); s0, s1: scales of the M0 and M1 tables
+inline void p2s_tables(int N, std::vector<_Float16>& out, int* s0, int* s1) {
+    const P2sFolded f = p2s_folded(N);
+    const int rs = N / 2 + 1, ra = N / 2;
+    std::vector<long double> all0(f.q0s), all1(f.q1s);
+    all0.insert(all0.end(), f.q0a.begin(), f.q0a.end());
+    all1.insert(all1.end(), f.q1a.begin(), f.q1a.end());
     *s0 = table_scale(all0);
     *s1 = table_scale(all1);
-    mfma_fragments(q0s, rs, N + 1, *s0, out);
-    mfma_fragments(q0a, ra, N, *s0, out);
-    mfma_fragments(q1s, rs, N, *s1, out);
-    mfma_fragments(q1a, ra, N + 1, *s1, out);
+    mfma_fragments(f.q0s, rs, N + 1, *s0, out);
+    mfma_fragments(f.q0a, ra, N, *s0, out);
+    mfma_fragments(f.q1s, rs, N, *s1, out);
+    mfma_fragments(f.q1a, ra, N + 1, *s1, out);
+}
+
+// Append the fragments of a rows x cols table (row-major) in the A-operand lane order of
+// v_mfma_f64_16x16x4_f64: fragment (row tile rt, k-step s) is [lane l] = T[16 rt + l % 16][4 s + l / 16]
+// (zero outside the table).
+inline void dmfma_fragments(const std::vector<long double>& T, int rows, int cols, std::vector<double>& out) {
+    const int RT = (rows + 15) / 16, KS = (cols + 3) / 4;
+    for (int rt = 0; rt < RT; rt++)
+        for (int s = 0; s < KS; s++)
+            for (int l = 0; l < 64; l++) {
+                const int r = 16 * rt + l % 16, k = 4 * s + l / 16;
+                out.push_back(r < rows && k < cols ? (double)T[(size_t)r * cols + k] : 0.0);
+            }
+}
+
+// FP64 fragments of the folded tables (k_p2s_dmfma): Q0s, Q0a, Q1s, Q1a, then the centre rows
+// Q0s[N/2][0..N] and Q1s[N/2][0..N-1] as plain doubles
+inline void p2s_tables_f64(int N, std::vector<double>& out) {
+    const P2sFolded f = p2s_folded(N);
+    const int rs = N / 2 + 1, ra = N / 2;
+    dmfma_fragments(f.q0s, rs, N + 1, out);
+    dmfma_fragments(f.q0a, ra, N, out);
+    dmfma_fragments(f.q1s, rs, N, out);
+    dmfma_fragments(f.q1a, ra, N + 1, out);
+    for (int q = 0; q <= N; q++) out.push_back((double)f.q0s[(size_t)(N / 2) * (N + 1) + q]);
+    for (int q = 0; q < N; q++) out.push_back((double)f.q1s[(size_t)(N / 2) * N + q]);
 }
 
 }  // namespace rpcf_host

@@ -27,6 +27,7 @@
 #include "k_linear.hpp"
 #include "k_transform.hpp"
 #include "k_cheb_mfma.hpp"
+#include "k_cheb_dmfma.hpp"
 
 // Static device scratch of the entry points that allocate nothing (include/rpcf_static.h):
 // each GPU holds its own copy, RPCF_STATIC_SCRATCH_BYTES, until the process ends.
@@ -88,6 +89,9 @@ struct rpcf_ctx {
     _Float16* MT = nullptr;  // s2p fragments, then p2s fragments
     long mt_p2s = 0;         // offset (halves) of the p2s fragments
     int mt_s0 = 0, mt_s1 = 0;
+    // FP64 form: the padded p2s with the nonlinear combination as FP64 products (k_cheb_dmfma.hpp)
+    int use_dmfma = 0;
+    double* DT = nullptr;  // fragments of the folded p2s tables
     // per-slot arrays ([capacity][...]) and their slot strides in elements
     double2* F = nullptr;     // [6][P][nz] u, v, w, ox, oy, oz
     double2* SV = nullptr;    // [P][nz] v-hat (u0 in pencil 0)
@@ -578,6 +582,29 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
                 cap);
         }
     }
+    if constexpr (f64) {
+        if (c->use_dmfma) {
+            const int nyb = (d.ny + cheb_dmfma::kPencils - 1) / cheb_dmfma::kPencils;
+            const int nitems = n * d.hnx * nyb;
+            return launch(
+                c, KID_P2S_NL,
+                [&] {
+                    switch (2 * d.N) {
+#define RPCF_CASE(L)                                                                                              \
+    case L:                                                                                                       \
+        if constexpr (L <= 384)                                                                                   \
+            k_p2s_dmfma<(L <= 384 ? L : 384)><<<cheb_dmfma::DShape<(L <= 384 ? L : 384)>::blocks(nitems),        \
+                                              cheb_dmfma::DShape<(L <= 384 ? L : 384)>::NT,                       \
+                                              cheb_dmfma::DShape<(L <= 384 ? L : 384)>::lds_bytes(), c->stream>>>( \
+                Y, yS, c->sY, c->NL, c->sNL, d, c->DT, mb, nitems, nyb);                                          \
+        break;
+                        RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+                    }
+                },
+                cap);
+        }
+    }
     if (c->TB_nl2) {
         const int TB = c->TB_nl2, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
@@ -634,7 +661,10 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
             const size_t lds_stream = sizeof(double) * 2 * c->tv_rt * c->tv_s * 64;
             const bool tv_stream = c->tv_s <= 17 && lds_stream <= (size_t)c->lds_limit;
             const int per_cu = lds_stream * 2 <= (size_t)c->lds_limit ? 2 : 1;
-            const int gs = std::max(1, std::min((d.P + 31) / 32, per_cu * c->ncu / (5 * n)));
+            int gs = std::max(1, std::min((d.P + 31) / 32, per_cu * c->ncu / (5 * n)));
+            // a multiple of 8: the five operator blocks of one pencil range (linear ids x + gs y)
+            // land on the same XCD (id mod 8), so v-hat and eta-hat come from its L2 after the first read
+            if (gs >= 16) gs &= ~7;
             switch (2 * d.N) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
@@ -700,7 +730,7 @@ int autotune_tiles(rpcf_ctx* c) {
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     // k_velocity_tile and k_p2s_nl2 run for compiled Chebyshev lengths only
-    const bool tunable[6] = {!c->use_mfma, true, true, true, c->TB_nl2 > 0 && !c->use_mfma, compiled};
+    const bool tunable[6] = {!c->use_mfma, true, true, true, c->TB_nl2 > 0 && !c->use_mfma && !c->use_dmfma, compiled};
     if (mode == "half") {
         for (int i = 0; i < 6; i++)
             if (tunable[i] && *tile_ref(c, i) >= 2) *tile_ref(c, i) /= 2;
@@ -782,7 +812,7 @@ void free_ctx(rpcf_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     void* ptrs[] = {c->F,   c->SV, c->SE, c->NL,  c->Y,     c->W,  c->Z2, c->WL, c->MINV, c->TF,  c->T4Q,
                     c->T2Q, c->U,  c->DU, c->DDU, c->MEANU, c->MP, c->PH, c->T0, c->T2,   c->T4,  c->T0t,
-                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->PAIRS, c->MT};
+                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->PAIRS, c->MT, c->DT};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -817,6 +847,18 @@ const MfmaTables& mfma_tables(int N) {
     t.p2s_offset = (long)t.h.size();
     rpcf_host::p2s_tables(N, t.h, &t.s0, &t.s1);
     return g_mt.emplace(N, std::move(t)).first->second;
+}
+
+// Host fragments of the FP64 p2s products of one length 2N, built once per process.
+std::map<int, std::vector<double>> g_dt;
+
+const std::vector<double>& dmfma_tables(int N) {
+    std::lock_guard<std::mutex> lk(g_mt_mu);
+    auto it = g_dt.find(N);
+    if (it != g_dt.end()) return it->second;
+    std::vector<double> t;
+    rpcf_host::p2s_tables_f64(N, t);
+    return g_dt.emplace(N, std::move(t)).first->second;
 }
 
 // allocation and shape set-up of a context of `capacity` slots; on failure the caller frees it
@@ -941,6 +983,21 @@ int init_ctx(rpcf_ctx* c) {
             c->use_mfma = 1;
         }
     }
+    // FP64 form: the padded p2s + nonlinear combination as FP64 matrix-core products for the
+    // compiled lengths up to 2N = 384; RPCF_P2S_MFMA=0 keeps the FFT-based kernels
+    if (!c->mixed && 2 * d.N <= 384) {
+        bool compiled = false;
+#define RPCF_CASE(L) compiled |= 2 * d.N == L;
+        RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+        const char* e = getenv("RPCF_P2S_MFMA");
+        if (compiled && !(e && e[0] == '0')) {
+            const std::vector<double>& t = dmfma_tables(d.N);
+            if ((rc = dalloc(c, &c->DT, t.size()))) return rc;
+            RCCHK(copy_sync(c, c->DT, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+            c->use_dmfma = 1;
+        }
+    }
 
     // LDS tiles, sized for three resident blocks per CU
     lim = c->lds_limit;
@@ -1017,7 +1074,8 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_velocity_tile<L>, lim);                   \
     allow_lds(k_velocity_tile<L, float2>, lim);           \
     allow_lds(k_p2s_mfma<(L <= 384 ? L : 384)>, lim);    \
-    allow_lds(k_s2p_mfma<(L <= 384 ? L : 384)>, lim);
+    allow_lds(k_s2p_mfma<(L <= 384 ? L : 384)>, lim);    \
+    allow_lds(k_p2s_dmfma<(L <= 384 ? L : 384)>, lim);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     allow_lds(k_cheb_p2s, lim);

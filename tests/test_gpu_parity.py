@@ -2,6 +2,8 @@
 seeded inputs, through the C ABI.  Metric: per field max|a-b|/max|b| over all pencils,
 the carried Nyquist pencils included.  Bars: 1e-12 after one step, 1e-9 after 100
 steps (FP64); matrices and velocity recovery bit for bit."""
+import os
+
 import numpy as np
 import pytest
 
@@ -215,3 +217,60 @@ def test_longest_compiled_chebyshev_step_parity():
     print("DEEP 5 steps", rel(a[0], b[0]), rel(a[1], b[1]))
     assert rel(a[0], b[0]) < 1e-11 and rel(a[1], b[1]) < 1e-11
     g.close(), o.close()
+
+
+# one grid per compiled Chebyshev length of the FP64 matrix-core p2s (2N = 24 .. 384)
+P2S_GRIDS = {
+    "2N24": (dict(mx=24, my=24, mz=48, n_pi_x=2.0, n_pi_y=1.0, Re=800.0, Ro=0.1, dt=0.002,
+                  u_profile=(0.0, 1.0, 0.0)), (0.05, 4, 6, 0.0, True)),
+    "2N64": (dict(mx=48, my=36, mz=128, n_pi_x=2.0, n_pi_y=1.0, Re=800.0, Ro=0.2, dt=0.005,
+                  u_profile=(1.0, 0.0, -1.0), flow_control=1), (0.05, 4, 8, 0.0, True)),
+    "2N96": (dict(mx=48, my=48, mz=192, n_pi_x=2.0, n_pi_y=1.0, Re=1500.0, Ro=0.3, dt=0.002,
+                  u_profile=(1.0, 0.0, -1.0), flow_control=1), (0.05, 6, 10, 0.0, True)),
+    "2N256": (None, None),  # the 128x129x128 channel
+    "2N384": (DEEP, (0.05, 8, 8, 0.0, True)),
+}
+
+
+def nonlinear_with(lib, prm, ic, p2s_mfma):
+    old = os.environ.get("RPCF_P2S_MFMA")
+    os.environ["RPCF_P2S_MFMA"] = "1" if p2s_mfma else "0"
+    try:
+        sim = rpcf.Simulation(lib, prm)
+    finally:
+        if old is None:
+            del os.environ["RPCF_P2S_MFMA"]
+        else:
+            os.environ["RPCF_P2S_MFMA"] = old
+    with sim:
+        sim.set_state(*rpcf.seed_state(sim.dll, prm, 3, *ic))
+        sim.nonlinear()
+        return sim.get_nonlinear()
+
+
+@pytest.mark.parametrize("name", list(P2S_GRIDS))
+def test_fp64_p2s_products_against_oracle(name):
+    """the nonlinear term with the padded p2s, derivative, rhsNonlinear and unpadded s2p as
+    FP64 matrix-core products (k_p2s_dmfma) against the FFT-based kernel on the same Lamb
+    components (N_eta within 1e-13) and against the oracle: N_eta within 1e-12, N_v no farther
+    from the oracle than the FFT-based kernel (plus 1e-13), which is within 5e-11.  The N_v bar is
+    wider than N_eta's because N_v = k^2 L_z + i(alpha dL_x + beta dL_y) multiplies the
+    round-off of the x/span FFTs (whose stage order differs from the oracle's) by k^2 and by
+    the derivative: its largest difference sits on the span Nyquist pencil (kx = 0,
+    ky = ny/2), 4.7e-13, 6.6e-12 and 1.5e-11 on the 2N = 24, 64, 96 grids, the same in both
+    kernels.  At 2N = 384 the derivative's own amplification (~N^2) shows: 3.6e-13 from the
+    oracle for the products, 5.6e-13 for the FFT-based kernel, 4.0e-13 between them"""
+    kw, ic = P2S_GRIDS[name]
+    prm = params("C1") if kw is None else rpcf.make_params(**kw)
+    ic = IC["C1"] if ic is None else ic
+    ov, oe = nonlinear_with(rpcf.ORACLE_LIB, prm, ic, False)
+    mv, me = nonlinear_with(rpcf.HIP_LIB, prm, ic, True)
+    fv, fe = nonlinear_with(rpcf.HIP_LIB, prm, ic, False)
+    errs = {"N_v": rel(mv, ov), "N_eta": rel(me, oe), "N_v (FFT)": rel(fv, ov), "N_eta (FFT)": rel(fe, oe),
+            "N_v MFMA vs FFT": rel(mv, fv), "N_eta MFMA vs FFT": rel(me, fe)}
+    at = np.unravel_index(np.argmax(np.abs(mv - ov)), mv.shape)
+    print(name, {k: "%.2e" % v for k, v in errs.items()}, "largest N_v difference at pencil, row", at)
+    assert errs["N_eta MFMA vs FFT"] < 1e-13
+    assert errs["N_eta"] < 1e-12 and errs["N_eta (FFT)"] < 1e-12
+    assert errs["N_v"] <= errs["N_v (FFT)"] + 1e-13 and errs["N_v (FFT)"] < 5e-11
+    assert not np.array_equal(mv, fv)  # both kernels ran
