@@ -359,8 +359,14 @@ __global__ __launch_bounds__(256) void k_span_inv(const cplx<R>* __restrict__ Y,
                 const unsigned kxi = (unsigned)i / my, s = i - kxi * my;
                 lds[kxi * ldl + s] = v;
             });
+        FftPlan pll = pl;  // compiled lengths: twiddles from an LDS copy behind the tile
+        if constexpr (FN > 0) {
+            C2* twl = lds + TK * ldl;
+            for (int i = tid; i < FN; i += nthr) twl[i] = ((const C2*)pl.tw)[i];
+            pll.tw = twl;
+        }
         __syncthreads();
-        fft_any<FN, +1, span_tile_max(FN)>(lds, ldl, nk, pl, tid, nthr);
+        fft_any<FN, +1, span_tile_max(FN)>(lds, ldl, nk, pll, tid, nthr);
         C2* out = W + f * wstride;
         staged<12>(
             my * nk, tid, nthr,
@@ -410,7 +416,8 @@ __device__ __forceinline__ void r2c_unpack(const C* z, int k, int mx, decltype(C
 // source/nonlinear.cu:400-403), x R2C of the three Lamb components and the
 // 1/(mx my) normalisation of source/transform.cu:267.  Member blockIdx.z (its Ro and
 // padded base profile meanU).  One block: one plane q, RP
-// pairs of span rows.  W[6][pz][my][nxh] -> Z2[3][pz][my][hnx].
+// pairs of span rows.  W[6][pz][my][nxh] -> Z2[3][pz][my][hnx].  LDS: [6 RP][ld] sequences,
+// then (compiled lengths) the FN twiddles.
 // -------------------------------------------------------------------------------------
 template <int FN = 0, typename R = double>  // R: precision of W, Z2, the FFTs and the products
 __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W, long wstride, long sW,
@@ -475,8 +482,17 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
             const int seq = i / nzero, k = i - seq * nzero;
             lds[seq * lds_ld + nxh + k] = cmake<C2>(0, 0);
         }
+        // compiled lengths: the FFT twiddles from an LDS copy behind the sequences, so that
+        // no butterfly waits on a global load
+        FftPlan pll = pl;
+        if constexpr (FN > 0) {
+            C2* twl = lds + 6 * RP * lds_ld;
+            const C2* twg = (const C2*)pl.tw;
+            for (int i = tid; i < FN; i += nthr) twl[i] = twg[i];
+            pll.tw = twl;
+        }
         __syncthreads();
-        fft_any<FN, +1, 6 * lamb_pairs_max(FN)>(lds, lds_ld, 6 * nrp, pl, tid, nthr);
+        fft_any<FN, +1, 6 * lamb_pairs_max(FN)>(lds, lds_ld, 6 * nrp, pll, tid, nthr);
         // Ro U(y) is uniform in x and span, so it reaches only the (0,0) pencil of L_z, which
         // rhsNonlinear does not use: the FP32 form leaves it out (in FP32 its round-off would
         // swamp the perturbation products); the FP64 form keeps the reference's expression.
@@ -494,7 +510,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
             lds[(2 * nrp + ip) * lds_ld + x] = cmake<C2>(lza, lzb);
         }
         __syncthreads();
-        fft_any<FN, -1, 3 * lamb_pairs_max(FN)>(lds, lds_ld, 3 * nrp, pl, tid, nthr);
+        fft_any<FN, -1, 3 * lamb_pairs_max(FN)>(lds, lds_ld, 3 * nrp, pll, tid, nthr);
         for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
             const int seq = i / hnx, k = i - seq * hnx;
             const int g = seq / nrp, ip = seq - g * nrp;
@@ -595,8 +611,14 @@ __global__ __launch_bounds__(256) void k_span_fwd(const cplx<R>* __restrict__ Z2
                 const unsigned s = (unsigned)i / nk, kxi = i - s * nk;
                 lds[kxi * ldl + s] = v;
             });
+        FftPlan pll = pl;  // compiled lengths: twiddles from an LDS copy behind the tile
+        if constexpr (FN > 0) {
+            C2* twl = lds + TK * ldl;
+            for (int i = tid; i < FN; i += nthr) twl[i] = ((const C2*)pl.tw)[i];
+            pll.tw = twl;
+        }
         __syncthreads();
-        fft_any<FN, -1, span_tile_max(FN)>(lds, ldl, nk, pl, tid, nthr);
+        fft_any<FN, -1, span_tile_max(FN)>(lds, ldl, nk, pll, tid, nthr);
         C2* out = Z1 + g * ostride;
         staged<8>(
             nk * ny, tid, nthr,

@@ -205,6 +205,15 @@ SlotSizes slot_sizes(const Dims& d, int precision) {
     return s;
 }
 
+// a transform length with a compile-time FFT plan
+bool compiled_len(int n) {
+    bool ok = false;
+#define RPCF_CASE(L) ok |= n == L;
+    RPCF_FFT_LENGTHS(RPCF_CASE)
+#undef RPCF_CASE
+    return ok;
+}
+
 hipEvent_t get_event(rpcf_ctx* c) {
     if (!c->pool.empty()) {
         hipEvent_t e = c->pool.back();
@@ -502,7 +511,7 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     {
         const int TK = c->TK_inv, ld = c->ld_my;
         dim3 g((d.nxh + TK - 1) / TK, d.pz, 6 * n);
-        size_t lds = sizeof(C2) * TK * ld;
+        size_t lds = sizeof(C2) * (TK * ld + (compiled_len(d.my) ? d.my : 0));
         rc = launch(
             c, KID_SPAN_INV,
             [&] {
@@ -522,7 +531,7 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     {
         const int RP = c->RP, ld = c->ld_mx;
         dim3 g((d.my / 2 + RP - 1) / RP, d.pz, n);
-        size_t lds = sizeof(C2) * 6 * RP * ld;
+        size_t lds = sizeof(C2) * (6 * RP * ld + (compiled_len(d.mx) ? d.mx : 0));
         rc = launch(
             c, KID_LAMB,
             [&] {
@@ -543,7 +552,7 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     {
         const int TK = c->TK_fwd, ld = c->ld_my;
         dim3 g((d.hnx + TK - 1) / TK, d.pz, 3 * n);
-        size_t lds = sizeof(C2) * TK * ld;
+        size_t lds = sizeof(C2) * (TK * ld + (compiled_len(d.my) ? d.my : 0));
         rc = launch(
             c, KID_SPAN_FWD,
             [&] {
@@ -1042,9 +1051,13 @@ int init_ctx(rpcf_ctx* c) {
     if (d.mx == L) rp_max = lamb_pairs_max(L);
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
-    c->TK_inv = pick_tile(tk_max, d.my, c->cap_y, lim3, [&](int t) { return sizeof(double2) * (size_t)t * c->ld_my; });
+    c->TK_inv = pick_tile(tk_max, d.my, c->cap_y, lim3, [&](int t) {
+        return sizeof(double2) * ((size_t)t * c->ld_my + (compiled_len(d.my) ? d.my : 0));
+    });
     c->TK_fwd = c->TK_inv;
-    c->RP = pick_tile(rp_max, 6 * d.mx, c->cap_x, lim3, [&](int t) { return sizeof(double2) * 6 * (size_t)t * c->ld_mx; });
+    c->RP = pick_tile(rp_max, 6 * d.mx, c->cap_x, lim3, [&](int t) {
+        return sizeof(double2) * (6 * (size_t)t * c->ld_mx + (compiled_len(d.mx) ? d.mx : 0));
+    });
     if (!c->TB_s2p || !c->TB_nl || !c->TK_inv || !c->RP)
         return fail(c, RPCF_EINVAL, "grid too large for the LDS tiles (limit %d bytes)", lim);
     allow_lds(k_cheb_s2p<1>, lim);
