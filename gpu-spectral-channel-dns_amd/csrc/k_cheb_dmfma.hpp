@@ -31,7 +31,13 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kPencils = 16;  // pencils per item: columns 0..15 real parts, 16..31 imaginary parts
 constexpr int kCols = 2 * kPencils;
-constexpr int kChunk = 32;    // samples q per chunk of the product's inner dimension (8 k-steps) at most
+#ifndef RPCF_DMFMA_CHUNK
+#define RPCF_DMFMA_CHUNK 32
+#endif
+#ifndef RPCF_DMFMA_WAVES
+#define RPCF_DMFMA_WAVES 3
+#endif
+constexpr int kChunk = RPCF_DMFMA_CHUNK;  // samples q per chunk of the product's inner dimension, at most
 
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
@@ -63,13 +69,13 @@ struct DShape {
     static constexpr long CENTRE_OFF = FEND * 64;
     static constexpr long table_doubles = CENTRE_OFF + (N + 1) + N;
     // LDS: the operand buffer [set h3, h2, h1][G+, G-][CHK][kCols] doubles, reused after the
-    // last chunk to stage the block's output rows [N_v, N_eta][pencil][SR][re, im]: rows p of
-    // its tiles, their mirrors N - p and the centre row
+    // last chunk to stage the block's output rows [pencil][SR][re, im] of N_v, then of N_eta:
+    // rows p of its tiles, their mirrors N - p and the centre row
     static constexpr int BUF = 3 * 2 * CHK * kCols;
     static constexpr int SR = 32 * TPB + 1;
     static constexpr size_t lds_bytes() {
         const size_t ops = (size_t)BUF * sizeof(double);
-        const size_t out = 2 * (size_t)kPencils * SR * 2 * sizeof(double);
+        const size_t out = (size_t)kPencils * SR * 2 * sizeof(double);
         return ops > out ? ops : out;
     }
     // blocks for nitems items: groups of 8 items, the H blocks of an item 8 ids apart (the
@@ -96,7 +102,7 @@ struct DShape {
 // centre row N/2 of its own runs on the vector units from the same LDS operands.
 // -------------------------------------------------------------------------------------
 template <int FN>
-__global__ __launch_bounds__(cheb_dmfma::DShape<FN>::NT, 3) void k_p2s_dmfma(
+__global__ __launch_bounds__(cheb_dmfma::DShape<FN>::NT, RPCF_DMFMA_WAVES) void k_p2s_dmfma(
     const double2* __restrict__ Z1, long zstride, long sY, double2* __restrict__ NL, long sNL, Dims d,
     const double* __restrict__ tab, Members mb, int nitems, int nyb) {
     using namespace cheb_dmfma;
@@ -240,75 +246,71 @@ __global__ __launch_bounds__(cheb_dmfma::DShape<FN>::NT, 3) void k_p2s_dmfma(
         }
         __syncthreads();  // the operands are read before the next chunk (or the stage) overwrites them
     }
-    // output stage [N_v, N_eta][pencil][SR][re, im]: local row j < 16 TPB is p = p0 + j, row
-    // 16 TPB + j its mirror N - p, row 32 TPB the centre
+    // output stage [pencil][SR][re, im], one output at a time (N_v, then N_eta): local row
+    // j < 16 TPB is p = p0 + j, row 16 TPB + j its mirror N - p, row 32 TPB the centre
     const int p0 = 16 * h * TPB;
-    double* outv = lds;
-    double* oute = lds + 2 * kPencils * SR;
+    double* out = lds;
+    double cen[2] = {0.0, 0.0};  // centre row sums of N_v, N_eta (threads tid < kCols)
     if (centre) {
         cE += __shfl_xor(cE, 32);
         cV += __shfl_xor(cV, 32);
         if (lane < 32) cpart[wave][0][lane] = cE, cpart[wave][1][lane] = cV;
     }
-    if (!anti) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int j = 16 * (wave) + kq + 4 * r, p = p0 + j;
-            if (p >= Sh::RS || (Sh::CENTRE && p == N / 2)) continue;
-#pragma unroll
-            for (int ct = 0; ct < 2; ct++) {
-                outv[(lc * SR + j) * 2 + ct] = accV[ct][r];
-                oute[(lc * SR + j) * 2 + ct] = accE[ct][r];
-                outv[(lc * SR + 16 * TPB + j) * 2 + ct] = accV[ct][r];
-                oute[(lc * SR + 16 * TPB + j) * 2 + ct] = accE[ct][r];
-            }
-        }
-    }
     __syncthreads();
-    if (anti) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int j = 16 * (wave - TPB) + kq + 4 * r, p = p0 + j;
-            if (p >= Sh::RA) continue;
-#pragma unroll
-            for (int ct = 0; ct < 2; ct++) {
-                outv[(lc * SR + j) * 2 + ct] += accV[ct][r];
-                oute[(lc * SR + j) * 2 + ct] += accE[ct][r];
-                outv[(lc * SR + 16 * TPB + j) * 2 + ct] -= accV[ct][r];
-                oute[(lc * SR + 16 * TPB + j) * 2 + ct] -= accE[ct][r];
-            }
-        }
-    }
     if (centre && tid < kCols) {
-        double e = 0.0, v = 0.0;
-        for (int w = 0; w < 2 * TPB; w++) e += cpart[w][0][tid], v += cpart[w][1][tid];
-        const int b = tid % kPencils, part = tid / kPencils;
-        outv[(b * SR + 32 * TPB) * 2 + part] = v;
-        oute[(b * SR + 32 * TPB) * 2 + part] = e;
+        for (int w = 0; w < 2 * TPB; w++) cen[1] += cpart[w][0][tid], cen[0] += cpart[w][1][tid];
     }
-    __syncthreads();
     const int ph = mb.phase[slot];
     const long PS = (long)d.P * nz;
-    double2* NLv = NL + slot * sNL + ph * PS;
-    double2* NLe = NL + slot * sNL + (2 + ph) * PS;
-    const double2* ov = (const double2*)outv;
-    const double2* oe = (const double2*)oute;
     const long po = ((long)ky0 * d.hnx + kx) * nz, pstride = (long)d.hnx * nz;
-    for (int i = tid; i < nb * SR; i += NT) {
-        const int bb = i / SR, j = i - bb * SR;
-        int row;
-        if (j < 16 * TPB) {
-            const int p = p0 + j;
-            row = p < Sh::RS && !(Sh::CENTRE && p == N / 2) ? p : -1;
-        } else if (j < 32 * TPB) {
-            const int p = p0 + j - 16 * TPB;
-            row = p < Sh::RA ? N - p : -1;
-        } else {
-            row = centre ? N / 2 : -1;
+#pragma unroll
+    for (int o = 0; o < 2; o++) {
+        const v4d* acc = o ? accE : accV;
+        // sym waves write S to rows p and N - p, then anti waves add A to row p and
+        // subtract it from row N - p
+        if (!anti) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int j = 16 * wave + kq + 4 * r, p = p0 + j;
+                if (p >= Sh::RS || (Sh::CENTRE && p == N / 2)) continue;
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) {
+                    out[(lc * SR + j) * 2 + ct] = acc[ct][r];
+                    out[(lc * SR + 16 * TPB + j) * 2 + ct] = acc[ct][r];
+                }
+            }
         }
-        if (row >= 0) {
-            NLv[po + bb * pstride + row] = ov[i];
-            NLe[po + bb * pstride + row] = oe[i];
+        if (centre && tid < kCols) out[((tid % kPencils) * SR + 32 * TPB) * 2 + tid / kPencils] = cen[o];
+        __syncthreads();
+        if (anti) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int j = 16 * (wave - TPB) + kq + 4 * r, p = p0 + j;
+                if (p >= Sh::RA) continue;
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) {
+                    out[(lc * SR + j) * 2 + ct] += acc[ct][r];
+                    out[(lc * SR + 16 * TPB + j) * 2 + ct] -= acc[ct][r];
+                }
+            }
         }
+        __syncthreads();
+        double2* dst = NL + slot * sNL + (o ? 2 + ph : ph) * PS;
+        const double2* ov = (const double2*)out;
+        for (int i = tid; i < nb * SR; i += NT) {
+            const int bb = i / SR, j = i - bb * SR;
+            int row;
+            if (j < 16 * TPB) {
+                const int p = p0 + j;
+                row = p < Sh::RS && !(Sh::CENTRE && p == N / 2) ? p : -1;
+            } else if (j < 32 * TPB) {
+                const int p = p0 + j - 16 * TPB;
+                row = p < Sh::RA ? N - p : -1;
+            } else {
+                row = centre ? N / 2 : -1;
+            }
+            if (row >= 0) dst[po + bb * pstride + row] = ov[i];
+        }
+        if (o == 0) __syncthreads();  // the stage is read before N_eta overwrites it
     }
 }

@@ -1014,8 +1014,9 @@ __global__ __launch_bounds__(256) void k_p2s_nl(const cplx<RI>* __restrict__ Z1,
 // LDS: [2*TB][ld] sequences, 256 scan totals, FN FFT twiddles, nz p2s twiddles.
 // -------------------------------------------------------------------------------------
 // three waves per SIMD (168 registers instead of 172 at two waves) up to 2N = 256; the
-// longer lengths keep more samples and coefficients per thread and run two waves (no spills)
-constexpr int p2s_waves(int fn) { return fn <= 256 ? 3 : 2; }
+// longer lengths keep more samples and coefficients per thread and run two waves, 2N = 512
+// one (no spills)
+constexpr int p2s_waves(int fn) { return fn <= 256 ? 3 : fn <= 384 ? 2 : 1; }
 template <int FN, typename RI = double>
 __global__ __launch_bounds__(256, p2s_waves(FN)) void k_p2s_nl2(const cplx<RI>* __restrict__ Z1, long zstride, long sY,
                                                  double2* __restrict__ NL, long sNL, Dims d, FftPlan plN,
