@@ -250,8 +250,9 @@ inline void dmfma_fragments(const std::vector<long double>& T, int rows, int col
             }
 }
 
-// FP64 fragments of the folded tables (k_p2s_dmfma): Q0s, Q0a, Q1s, Q1a, then the centre rows
-// Q0s[N/2][0..N] and Q1s[N/2][0..N-1] as plain doubles
+// FP64 fragments of the folded tables (k_p2s_dmfma): Q0s, Q0a, Q1s, Q1a, then as plain doubles
+// the centre rows Q0s[N/2][0..N] and Q1s[N/2][0..N-1] and the columns of the centre sample
+// Q0s[0..N/2][N] and Q1a[0..N/2-1][N]
 inline void p2s_tables_f64(int N, std::vector<double>& out) {
     const P2sFolded f = p2s_folded(N);
     const int rs = N / 2 + 1, ra = N / 2;
@@ -261,6 +262,8 @@ inline void p2s_tables_f64(int N, std::vector<double>& out) {
     dmfma_fragments(f.q1a, ra, N + 1, out);
     for (int q = 0; q <= N; q++) out.push_back((double)f.q0s[(size_t)(N / 2) * (N + 1) + q]);
     for (int q = 0; q < N; q++) out.push_back((double)f.q1s[(size_t)(N / 2) * N + q]);
+    for (int p = 0; p < rs; p++) out.push_back((double)f.q0s[(size_t)p * (N + 1) + N]);
+    for (int p = 0; p < ra; p++) out.push_back((double)f.q1a[(size_t)p * (N + 1) + N]);
 }
 
 }  // namespace rpcf_host
