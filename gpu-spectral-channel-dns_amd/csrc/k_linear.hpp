@@ -466,8 +466,8 @@ __global__ __launch_bounds__(256, tv_waves(S)) void k_lin_tv_mfma(const double2*
 // operator), the next group's even and odd coefficients loaded while the current group's
 // RT x 2 chains of S products run (one block, one wave per SIMD per compute unit).  The table is read from L2 once per block instead of once
 // per 32 pencils.  Same tiles and summation order as k_lin_tv_mfma: bit-identical results.
-template <int S>
-__global__ __launch_bounds__(256) void k_lin_tv_stream(const double2* __restrict__ sv, const double2* __restrict__ se,
+template <int S, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void k_lin_tv_stream(const double2* __restrict__ sv, const double2* __restrict__ se,
                                                        long sS, double2* __restrict__ W, long sWl, Dims d,
                                                        const double* __restrict__ Tf, long sTf, int RT,
                                                        const int* __restrict__ ids) {
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(256) void k_lin_tv_stream(const double2* __restrict
     }
     __syncthreads();
     const int n = d.nz, N = d.N, col = lane & 15, kq = lane >> 4, rows = N / 2 + 1;
-    const int ngroups = (d.P + 7) / 8, stride = gridDim.x * 4;
+    const int ngroups = (d.P + 7) / 8, stride = gridDim.x * NW;
     const double2* x = m < 3 ? sv : se;
     double* Wm0 = (double*)(W + (long)m * d.P * n) + (col & 1);
     double be[S], bo[S];
@@ -502,13 +502,20 @@ __global__ __launch_bounds__(256) void k_lin_tv_stream(const double2* __restrict
             bo[s] = xp[2 * min(jo, n - 1)] * ((okp && jo < n) ? 1.0 : 0.0);
         }
     };
-    int g = blockIdx.x * 4 + wave;
-    load(g);
+    int g = blockIdx.x * NW + wave;
+    if constexpr (NW == 4) load(g);
     for (; g < ngroups; g += stride) {
         double ce[S], co[S];
+        if constexpr (NW == 4) {
 #pragma unroll
-        for (int s = 0; s < S; s++) ce[s] = be[s], co[s] = bo[s];
-        load(g + stride);  // the next group's coefficients, in flight during this group's products
+            for (int s = 0; s < S; s++) ce[s] = be[s], co[s] = bo[s];
+            load(g + stride);  // the next group's coefficients, in flight during this group's products
+        } else {
+            // two waves per SIMD: the other wave's products hide this group's loads
+            load(g);
+#pragma unroll
+            for (int s = 0; s < S; s++) ce[s] = be[s], co[s] = bo[s];
+        }
         const int p = g * 8 + (col >> 1);
         double* Wm = Wm0 + (long)p * 2 * n;
         // a last row tile that holds only the centre row p = N/2 (N/2 a multiple of 16) is a
@@ -529,11 +536,18 @@ __global__ __launch_bounds__(256) void k_lin_tv_stream(const double2* __restrict
             if (kq == 0 && p < d.P) Wm[2 * (N / 2)] = e + o;
         }
         for (int t = 0; t < RTm; t++) {
+            // the row tile's table fragments first, so that no product waits on an LDS read
+            double fe[S], fo[S];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                fe[s] = tvT[(t * S + s) * 64 + lane];
+                fo[s] = tvT[((RT + t) * S + s) * 64 + lane];
+            }
             rpcf_v4d ae = {0.0, 0.0, 0.0, 0.0}, ao = ae;
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                ae = __builtin_amdgcn_mfma_f64_16x16x4f64(tvT[(t * S + s) * 64 + lane], ce[s], ae, 0, 0, 0);
-                ao = __builtin_amdgcn_mfma_f64_16x16x4f64(tvT[((RT + t) * S + s) * 64 + lane], co[s], ao, 0, 0, 0);
+                ae = __builtin_amdgcn_mfma_f64_16x16x4f64(fe[s], ce[s], ae, 0, 0, 0);
+                ao = __builtin_amdgcn_mfma_f64_16x16x4f64(fo[s], co[s], ao, 0, 0, 0);
             }
             if (p < d.P) {
 #pragma unroll

@@ -678,7 +678,7 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
         if (tv_stream)                                                                                           \
-            k_lin_tv_stream<(L / 4 + 4) / 4><<<dim3(gs, 5 * n), 256, lds_stream, c->stream>>>(                    \
+            k_lin_tv_stream<(L / 4 + 4) / 4, 8><<<dim3(gs, 5 * n), 512, lds_stream, c->stream>>>(                 \
                 c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, c->sTf, c->tv_rt, ids);                            \
         else                                                                                                     \
             k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d,    \
@@ -1074,7 +1074,7 @@ int init_ctx(rpcf_ctx* c) {
     allow_lds(k_lamb_phys<L>, lim);   \
     allow_lds(k_p2s_nl<L>, lim);      \
     allow_lds(k_lin_tv_mfma<(L / 4 + 4) / 4>, lim);      \
-    allow_lds(k_lin_tv_stream<(L / 4 + 4) / 4>, lim);    \
+    allow_lds(k_lin_tv_stream<(L / 4 + 4) / 4, 8>, lim); \
     allow_lds(k_cheb_s2p<1, L, float>, lim);              \
     allow_lds(k_span_inv<L, float>, lim);                 \
     allow_lds(k_span_fwd<L, float>, lim);                 \
