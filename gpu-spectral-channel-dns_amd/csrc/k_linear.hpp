@@ -571,6 +571,11 @@ __global__ __launch_bounds__(64 * NW) void k_lin_tv_stream(const double2* __rest
 // (double).  The rows land in threads tid < n; every thread of the block calls it (it
 // synchronises).
 constexpr int kSolveThreads = 576;
+// matrix loads per thread in flight in the solve's GEMVs: with 4 the solve needs 80 registers
+// and no spills, six waves per SIMD; at 128x129x128 the parity-block solve takes 282 us
+// against 299 with 8 (its fixed phases weigh more), the full-inverse solve the same 474 us.
+// The summation order is the same for any batch.
+constexpr int kGemvBatch = 4;
 __device__ __forceinline__ void gemv_acc(double2 m, double2 u0, double2 u1, double* a) {
     a[0] = fma(m.x, u0.x, fma(-m.y, u0.y, a[0]));
     a[1] = fma(m.x, u0.y, fma(m.y, u0.x, a[1]));
@@ -589,7 +594,7 @@ __device__ __forceinline__ void gemv_acc(double c, double2 u0, double2 u1, doubl
 template <typename MT>
 __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const double2* x0, const double2* x1, int n,
                                              double2* part, int tid, int nthr, double2* y0, double2* y1, int j0 = 0) {
-    constexpr int B = 8;
+    constexpr int B = kGemvBatch;
     const int G = nthr / n;
     const int g = tid / n, r = tid - g * n;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
@@ -634,7 +639,7 @@ __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const dou
 __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const double2* xa0, const double2* xa1,
                                          const double2* xb0, const double2* xb1, int rows, int cols, int split,
                                          double2* part, int tid, int nthr, double2* y0, double2* y1, int j0) {
-    constexpr int B = 8;
+    constexpr int B = kGemvBatch;
     const int G = nthr / rows;
     const int g = tid / rows, r = tid - g * rows;
     const double2* x0 = r < split ? xa0 : xb0;
@@ -722,7 +727,7 @@ __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __res
 __device__ __forceinline__ void gemv_t0_reflect(const double* __restrict__ T0t, const double2* x0, const double2* x1,
                                                 int n, int N, double2* part, int tid, int nthr, double2* y0,
                                                 double2* y1) {
-    constexpr int B = 8;
+    constexpr int B = kGemvBatch;
     const int h = N / 2 + 1, G = (nthr / h) & ~1;
     const int g = tid / h, r = tid - g * h;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
@@ -804,7 +809,7 @@ __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, 
 //   v rows r = 4..N, p = r-2:  (T4q v + k2 T2 v - k0 T0 v) + i (a2 T2 v + a0 T0 v)
 //   eta rows r = 2..N, p = r-1: (T2q eta + c0 T0 eta) + i ci T0 eta + i e T0 (v^n + v^{n+1})
 // -------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kSolveThreads, 5) void k_lin_solve(double2* __restrict__ SV, double2* __restrict__ SE, long sS,
+__global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restrict__ SV, double2* __restrict__ SE, long sS,
                                                    const double2* __restrict__ NL, long sNL,
                                                    const double2* __restrict__ MINV, long sM,
                                                    const double2* __restrict__ MINVH, long sMH,
