@@ -99,9 +99,10 @@ struct DShape {
 // x, y, z (loaded one chunk ahead) into G+ and G-, combine them into h3, h2, h1 and write them
 // as B operands to LDS; the waves load their table fragments (from L2); after a barrier each
 // wave runs the chunk's k-steps for its tile over both column tiles -- Q0 fragments serve h3
-// and h1, Q1 fragments h2 -- while the next chunk's samples are in flight.  The centre sample
-// q = N enters afterwards as a rank-one update of the accumulators on the vector units.  Three blocks
-// share a compute unit, so one block's products overlap another's loads.  At the end the
+// and h1, Q1 fragments h2 -- while the next chunk's samples are in flight.  The centre
+// sample q = N enters afterwards as a rank-one update of the accumulators on the vector
+// units.  Three blocks share a compute unit, so one block's products overlap another's
+// loads.  At the end the
 // sym waves write S to rows p and N - p of an LDS stage, the anti waves add A to row p and
 // subtract it from row N - p, and the rows leave as contiguous runs of the pencils.  A
 // centre row N/2 of its own runs on the vector units from the same LDS operands.
