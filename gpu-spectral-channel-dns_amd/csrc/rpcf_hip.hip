@@ -72,6 +72,10 @@ struct rpcf_ctx {
     double fac = 0.0;
     int device = 0;
     hipStream_t stream = nullptr;
+    // the operator products of a step run on a second stream, beside the nonlinear transforms
+    // (they depend on the state only): fork after the state is final, join before the solve
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int lds_limit = 65536;
     int ncu = 256;                   // compute units of the device
     long matrix_cache_hits = 0;      // slots whose inverses were copied from another slot
@@ -658,11 +662,11 @@ int enq_nonlinear(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool c
     return c->mixed ? enq_transforms<float>(c, ids, n, cap) : enq_transforms<double>(c, ids, n, cap);
 }
 
-int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
+// The operator products of the linear step on stream s (the context stream, timed when
+// asked, or the side stream, untimed).
+int enq_lin_tv(rpcf_ctx* c, const int* ids, int n, bool cap, hipStream_t s) {
     const Dims d = c->d;
-    int rc = launch(
-        c, KID_LIN_TV,
-        [&] {
+    auto body = [&] {
             const size_t lds = sizeof(double) * 2 * kTvChunk * c->tv_s * 64;
             dim3 g((d.P + kTvBlockPencils - 1) / kTvBlockPencils, 5 * n);
             // persistent blocks with an operator's whole table in LDS when it fits (2N <= 256):
@@ -678,24 +682,30 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
 #define RPCF_CASE(L)                                                                                             \
     case L:                                                                                                      \
         if (tv_stream)                                                                                           \
-            k_lin_tv_stream<(L / 4 + 4) / 4, 8><<<dim3(gs, 5 * n), 512, lds_stream, c->stream>>>(                 \
+            k_lin_tv_stream<(L / 4 + 4) / 4, 8><<<dim3(gs, 5 * n), 512, lds_stream, s>>>(                         \
                 c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->TF, c->sTf, c->tv_rt, ids);                            \
         else                                                                                                     \
-            k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d,    \
-                                                                       c->TF, c->sTf, c->tv_rt, ids);           \
+            k_lin_tv_mfma<(L / 4 + 4) / 4><<<g, 256, lds, s>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d,            \
+                                                               c->TF, c->sTf, c->tv_rt, ids);                   \
         break;
                 RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                 default: {
                     dim3 gv((d.P + kTvPencils - 1) / kTvPencils, n);
                     const size_t lv = sizeof(double2) * 2 * (size_t)d.nz * (kTvPencils + 1);
-                    k_lin_tv<kTvPencils><<<gv, 384, lv, c->stream>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->T0t,
-                                                                     c->T2t, c->T4Q, c->T2Q, c->sQ, ids);
+                    k_lin_tv<kTvPencils><<<gv, 384, lv, s>>>(c->SV, c->SE, c->sS, c->WL, c->sWl, d, c->T0t,
+                                                             c->T2t, c->T4Q, c->T2Q, c->sQ, ids);
                 }
             }
-        },
-        cap);
-    if (rc) return rc;
+    };
+    if (s == c->stream) return launch(c, KID_LIN_TV, body, cap);
+    body();
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RPCF_OK : fail(c, RPCF_EDEVICE, "launch k_lin_tv: %s", hipGetErrorString(e));
+}
+
+int enq_lin_solve(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
+    const Dims d = c->d;
     return launch(
         c, KID_LINEAR,
         [&] {
@@ -707,6 +717,34 @@ int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
                                                                  c->DDU, c->sU, c->qw);
         },
         cap);
+}
+
+int enq_linear(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
+    int rc = enq_lin_tv(c, ids, n, cap, c->stream);
+    return rc ? rc : enq_lin_solve(c, ids, n, cap);
+}
+
+// One step of the listed slots: the nonlinear evaluation (with the velocity of the state when
+// with_velocity), the operator products and the solves.  The operator products read only the
+// state, so unless their launches are being timed they run on the side stream, forked after
+// the state is final and joined before the solve, beside the nonlinear transforms.
+int enq_step(rpcf_ctx* c, const int* ids, int n, bool with_velocity, bool cap = false) {
+    const bool side = !(c->timing && ((c->timing_mask >> KID_LIN_TV) & 1u));
+    if (!side) {
+        int rc = enq_nonlinear(c, ids, n, with_velocity, cap);
+        return rc ? rc : enq_linear(c, ids, n, cap);
+    }
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    int rc = enq_lin_tv(c, ids, n, cap, c->side);
+    // the join is recorded even after a failed launch, so that a capture is not left open
+    const hipError_t e1 = hipEventRecord(c->ev_join, c->side);
+    if (!rc) rc = enq_nonlinear(c, ids, n, with_velocity, cap);
+    const hipError_t e2 = hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    if (rc) return rc;
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return fail(c, RPCF_EDEVICE, "step fork/join: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    return enq_lin_solve(c, ids, n, cap);
 }
 
 // ---- tile autotuner ------------------------------------------------------------------
@@ -826,6 +864,9 @@ void free_ctx(rpcf_ctx* c) {
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : c->pool) (void)hipEventDestroy(e);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -879,6 +920,9 @@ int init_ctx(rpcf_ctx* c) {
     HIPCHK(c, hipGetDevice(&c->device));
     RCCHK(scratch_setup(c));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     int lim = 0;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lim > 0)
         c->lds_limit = lim;
@@ -1269,8 +1313,7 @@ int group_steps(rpcf_ctx* c, const std::vector<int>& ids, int nsteps) {
         }
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        rc = enq_nonlinear(c, c->ids_dev, n, true, true);
-        if (!rc) rc = enq_linear(c, c->ids_dev, n, true);
+        rc = enq_step(c, c->ids_dev, n, true, true);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
@@ -1289,8 +1332,7 @@ int group_steps(rpcf_ctx* c, const std::vector<int>& ids, int nsteps) {
         if (graph) {
             HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
         } else {
-            rc = enq_nonlinear(c, c->ids_dev, n, true);
-            if (!rc) rc = enq_linear(c, c->ids_dev, n);
+            rc = enq_step(c, c->ids_dev, n, true);
             if (rc) return rc;
         }
         for (int id : ids) c->slots[id].phase ^= 1;
@@ -1476,8 +1518,7 @@ int rpcf_step(rpcf_ctx* c, int nsteps) {
     // nextStep + getUVW: the velocity of the new state is formed when it is next needed
     // (by the next nonlinear evaluation, or by rpcf_get_field)
     for (int s = 0; s < nsteps; s++) {
-        rc = enq_nonlinear(c, c->ids_zero, 1, sh.fields == FIELDS_STALE);
-        if (!rc) rc = enq_linear(c, c->ids_zero, 1);
+        rc = enq_step(c, c->ids_zero, 1, sh.fields == FIELDS_STALE);
         if (rc) return rc;
         sh.phase ^= 1;
         sh.fields = FIELDS_STALE;
