@@ -48,15 +48,18 @@ def device_sync():
 
 def kernel_bytes(s, prm):
     """algorithmic HBM bytes per launch of each kernel: the arrays it must read and write
-    once.  FP64 form: complex FP64 (16 B) throughout, solves with the full inverses.  Mixed
-    form (precision 1; channel profile, matrix-core Chebyshev transforms): the velocity and
-    vorticity fields, the padded intermediates and the Lamb vector in complex FP32 (8 B), the
-    state, nonlinear terms and operator outputs in FP64, and the solves with the two parity
-    blocks of each inverse (nz x (N/2 + 1) complex entries).  Precision 2: FP64 with the
-    parity blocks."""
+    once.  FP64 form: complex FP64 (16 B) throughout; with an even base profile (the channel)
+    the solves read the two parity blocks of each inverse (nz x (N/2 + 1) complex FP64
+    entries) and their remainders (the same count of complex BF16 pairs, 4 B), otherwise
+    the full inverses (nz x nz).  Mixed form (precision 1; channel profile, matrix-core
+    Chebyshev transforms): the velocity and vorticity fields, the padded intermediates and
+    the Lamb vector in complex FP32 (8 B), the state, nonlinear terms and operator outputs in
+    FP64, and the solves with the parity blocks alone.  Precision 2: FP64 with the parity
+    blocks alone.  (The mean-flow slot reads its two full inverses in every form: 0.2 MB
+    more at 128x129x128, not counted.)"""
     P, nz, hnx, ny, pz = s["P"], s["nz"], s["hnx"], s["ny"], s["pz"]
     my, nxh = prm.my, s["nx"] // 2
-    parity = prm.precision != 0 and prm.u_profile[1] == 0.0
+    parity = prm.u_profile[1] == 0.0 and os.environ.get("RPCF_FULL_INVERSE", "0") == "0"
     mixed = prm.precision == 1 and 2 * (nz - 1) <= 384 and os.environ.get("RPCF_MIXED_MFMA", "1") != "0"
     h = 0.5 if prm.precision == 1 else 1.0  # FP32 intermediates of the mixed form
     S = P * nz * 16                    # one spectral field
@@ -65,8 +68,8 @@ def kernel_bytes(s, prm):
     Wb = pz * my * nxh * 16 * h        # one field after the span transform (kx < nx/2)
     Zb = pz * my * hnx * 16 * h        # one Lamb component after the x transform
     M1 = nz * nz * 16                  # one inverse matrix
-    if parity:
-        M1 = nz * ((nz - 1) // 2 + 1) * 16  # its parity blocks: nz rows x (N/2 + 1) pair columns
+    if parity:  # its parity blocks: nz rows x (N/2 + 1) pair columns, in FP64 with BF16 remainders
+        M1 = nz * ((nz - 1) // 2 + 1) * (16 + (4 if prm.precision == 0 else 0))
     nslot = hnx * (ny // 2 + 1)        # matrix slots: pencils +-ky share one
     out = {
         "k_cheb_s2p": 6 * F + 6 * Sp,

@@ -267,9 +267,10 @@ struct LinState {
     double2* se;          // eta-hat (w0 in pencil 0)
     const double2* minv;  // [2][nslot][nz*nz] column-major inverses
     const double2* minvh; // [2][nslot][np][nz] their parity blocks (k_parity_blocks)
+    const unsigned* minve; // [2][nslot][np][nz] the blocks' remainders, BF16 (re, im) pairs
     double qw_target_re, qw_target_im;  // flow-rate target (flow_control)
     int flow_control;
-    int parity;           // 1: solve with the parity blocks
+    int parity;           // 1: solve with the parity blocks, 2: with the blocks and remainders
 };
 
 // -------------------------------------------------------------------------------------
@@ -634,36 +635,63 @@ __device__ __forceinline__ void gemv_grouped(const MT* __restrict__ M, const dou
     }
 }
 
+// A complex BF16 pair (re in the low half) as a double2, exactly.
+__device__ __forceinline__ double2 bf16x2_to_double2(unsigned w) {
+    return make_double2((double)__uint_as_float(w << 16), (double)__uint_as_float(w & 0xffff0000u));
+}
+
 // The same with a rows x cols matrix (column-major, M[c*rows + r]) whose rows r < split
-// take the vectors xa and the others xb; columns from j0.
-__device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const double2* xa0, const double2* xa1,
-                                         const double2* xb0, const double2* xb1, int rows, int cols, int split,
-                                         double2* part, int tid, int nthr, double2* y0, double2* y1, int j0) {
+// take the vectors xa and the others xb; columns from j0.  With CORR the remainders E (same
+// layout, BF16 pairs) multiply the other vectors (rows r < split xb, the others xa), and
+// each column adds M x + E x' in that order.
+template <bool CORR>
+__device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const unsigned* __restrict__ E,
+                                         const double2* xa0, const double2* xa1, const double2* xb0,
+                                         const double2* xb1, int rows, int cols, int split, double2* part, int tid,
+                                         int nthr, double2* y0, double2* y1, int j0) {
     constexpr int B = kGemvBatch;
     const int G = nthr / rows;
     const int g = tid / rows, r = tid - g * rows;
     const double2* x0 = r < split ? xa0 : xb0;
     const double2* x1 = r < split ? xa1 : xb1;
+    const double2* o0 = r < split ? xb0 : xa0;
+    const double2* o1 = r < split ? xb1 : xa1;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     if (g < G) {
         int j = j0 + g;
         for (; j + G * (B - 1) < cols; j += G * B) {
             double2 m[B];
+            unsigned e[B];
 #pragma unroll
-            for (int u = 0; u < B; u++) m[u] = M[(long)(j + G * u) * rows + r];
+            for (int u = 0; u < B; u++) {
+                m[u] = M[(long)(j + G * u) * rows + r];
+                if (CORR) e[u] = E[(long)(j + G * u) * rows + r];
+            }
 #pragma unroll
-            for (int u = 0; u < B; u++) gemv_acc(m[u], x0[j + G * u], x1[j + G * u], a);
+            for (int u = 0; u < B; u++) {
+                gemv_acc(m[u], x0[j + G * u], x1[j + G * u], a);
+                if (CORR) gemv_acc(bf16x2_to_double2(e[u]), o0[j + G * u], o1[j + G * u], a);
+            }
         }
         if (j < cols) {
             double2 m[B];
+            unsigned e[B];
 #pragma unroll
-            for (int u = 0; u < B; u++) m[u] = M[(long)min(j + G * u, cols - 1) * rows + r];
+            for (int u = 0; u < B; u++) {
+                m[u] = M[(long)min(j + G * u, cols - 1) * rows + r];
+                if (CORR) e[u] = E[(long)min(j + G * u, cols - 1) * rows + r];
+            }
 #pragma unroll
             for (int u = 0; u < B; u++) {
                 const int jj = min(j + G * u, cols - 1);
                 const double k = j + G * u < cols ? 1.0 : 0.0;
                 const double2 u0 = x0[jj], u1 = x1[jj];
                 gemv_acc(m[u], make_double2(u0.x * k, u0.y * k), make_double2(u1.x * k, u1.y * k), a);
+                if (CORR) {
+                    const double2 w0 = o0[jj], w1 = o1[jj];
+                    gemv_acc(bf16x2_to_double2(e[u]), make_double2(w0.x * k, w0.y * k),
+                             make_double2(w1.x * k, w1.y * k), a);
+                }
             }
         }
         part[tid] = make_double2(a[0], a[1]);
@@ -694,27 +722,46 @@ __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const do
 // with H[j][p] = (M[j][i] + (-1)^j s M[j][i'])/2 -- half the bytes of M.  Computed from the
 // full inverse, so the blocks agree with it to its own (round-off) asymmetry.  Layout of a
 // block: np columns (pairs) of nz rows, the even j = 0, 2, .. first, then the odd j.
+//
+// The remainders E[j][p] = (M[j][i] - (-1)^j s M[j][i'])/2 (for the centre row: M[j][i] at
+// odd j, 0 at even j) restore the full inverse: H + E = M[j][i], H - E = (-1)^j s M[j][i'],
+// so  x_j = sum_p H[j][p] (b_i + (-1)^j s b_i') + E[j][p] (b_i - (-1)^j s b_i')  is M b
+// exactly.  E is the inverse's round-off asymmetry (~1e-14 of its largest entry at
+// 128x129x128, the difference of two nearly equal doubles, so computed exactly) and is
+// stored as BF16: its 2^-9 relative rounding is below the FP64 rounding of M itself, and
+// dropping E would move a solve by up to ~1e-12 (relative), so the blocks with their
+// remainders (5/8 of the bytes of M) keep the full inverse's results.  minve may be null.
 // -------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned bf16_bits(double x) {
+    unsigned u = __float_as_uint((float)x);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return u >> 16;
+}
 __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __restrict__ minvh,
-                                const int4* __restrict__ pairs, int np, Dims d) {
+                                unsigned* __restrict__ minve, const int4* __restrict__ pairs, int np, Dims d) {
     const int slot = blockIdx.x, eq = blockIdx.y, n = d.nz, ne = d.N / 2 + 1;
     const double2* M = minv + ((long)eq * d.nslot + slot) * n * n;  // M[j][i] at i*n + j
     double2* Hm = minvh + ((long)eq * d.nslot + slot) * (long)n * np;
+    unsigned* Em = minve ? minve + ((long)eq * d.nslot + slot) * (long)n * np : nullptr;
     const int4* pr = pairs + eq * np;
     for (long e = threadIdx.x; e < (long)n * np; e += blockDim.x) {
         const int p = (int)(e / n), r = (int)(e - (long)p * n);
         const int4 q = pr[p];
         const bool even = r < ne;
         const int j = even ? 2 * r : 2 * (r - ne) + 1;
-        double2 v;
+        double2 v, w;
         if (q.x == q.y) {
-            v = even ? M[(long)q.x * n + j] : make_double2(0.0, 0.0);
+            const double2 a = M[(long)q.x * n + j];
+            v = even ? a : make_double2(0.0, 0.0);
+            w = even ? make_double2(0.0, 0.0) : a;
         } else {
             const double2 a = M[(long)q.x * n + j], b = M[(long)q.y * n + j];
             const double sg = even ? (double)q.z : -(double)q.z;
             v = make_double2(0.5 * (a.x + sg * b.x), 0.5 * (a.y + sg * b.y));
+            w = make_double2(0.5 * (a.x - sg * b.x), 0.5 * (a.y - sg * b.y));
         }
         Hm[e] = v;
+        if (Em) Em[e] = bf16_bits(w.x) | (bf16_bits(w.y) << 16);
     }
 }
 
@@ -766,12 +813,13 @@ __device__ __forceinline__ void gemv_t0_reflect(const double* __restrict__ T0t, 
 // x = M b for the two right-hand sides rhs[0..1][nz] of a slot: with the full inverse M, or,
 // for a member with an even base profile, with its parity blocks H (k_parity_blocks): the
 // combined sides b_i + s b_i' and b_i - s b_i' per row pair, then one nz x np GEMV whose even
-// rows take the first and odd rows the second.  b vanishes on the rows below j0 (an even
+// rows take the first and odd rows the second (parity 2: and the remainders E the other).  b vanishes on the rows below j0 (an even
 // count: whole row pairs), whose columns are skipped.  Thread r < nz returns x at
 // coefficient *jrow.  Every thread of the block calls it (it synchronises).
-__device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const int4* pairs, int np, int parity,
-                                           double2 (*rhs)[256], double2 (*bc)[2][128], double2* part, int n, int N,
-                                           int tid, int nthr, double2* y0, double2* y1, int* jrow, int j0) {
+__device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const unsigned* Em, const int4* pairs,
+                                           int np, int parity, double2 (*rhs)[256], double2 (*bc)[2][128],
+                                           double2* part, int n, int N, int tid, int nthr, double2* y0, double2* y1,
+                                           int* jrow, int j0) {
     if (!parity) {
         gemv_grouped(M, rhs[0], rhs[1], n, part, tid, nthr, y0, y1, j0);
         *jrow = tid;
@@ -793,7 +841,10 @@ __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, 
     }
     __syncthreads();
     const int ne = N / 2 + 1;
-    gemv_sel(Hm, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
+    if (parity == 2)
+        gemv_sel<true>(Hm, Em, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
+    else
+        gemv_sel<false>(Hm, Em, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
     *jrow = tid < ne ? 2 * tid : 2 * (tid - ne) + 1;
 }
 
@@ -813,6 +864,7 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
                                                    const double2* __restrict__ NL, long sNL,
                                                    const double2* __restrict__ MINV, long sM,
                                                    const double2* __restrict__ MINVH, long sMH,
+                                                   const unsigned* __restrict__ MINVE, long sME,
                                                    const int4* __restrict__ pairs, int np,
                                                    const double2* __restrict__ W, long sWl, Dims d, Members mb,
                                                    const double* __restrict__ T0t, const double* __restrict__ U,
@@ -837,6 +889,7 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
         st.se = SE + ms * sS;
         st.minv = MINV + ms * sM;
         st.minvh = MINVH + ms * sMH;
+        st.minve = MINVE + ms * sME;
         st.parity = mp.parity;
         st.qw_target_re = mp.q_re;
         st.qw_target_im = mp.q_im;
@@ -935,8 +988,8 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
     {
         double2 y0, y1;
         int j;
-        solve_slot(Mv, st.minvh + (long)slot * nh, pairs, np, st.parity, rhs, bc, &part[0][0], n, d.N, r,
-                   blockDim.x, &y0, &y1, &j, 4);
+        solve_slot(Mv, st.minvh + (long)slot * nh, st.minve + (long)slot * nh, pairs, np, st.parity, rhs, bc,
+                   &part[0][0], n, d.N, r, blockDim.x, &y0, &y1, &j, 4);
         if (r < n) vnew[0][j] = y0, vnew[1][j] = y1;
     }
     __syncthreads();
@@ -965,8 +1018,8 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
     __syncthreads();
     double2 e0, e1;
     int je;
-    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, pairs + np, np, st.parity, rhs, bc, &part[0][0], n, d.N,
-               r, blockDim.x, &e0, &e1, &je, 2);
+    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, st.minve + ((long)d.nslot + slot) * nh, pairs + np, np,
+               st.parity, rhs, bc, &part[0][0], n, d.N, r, blockDim.x, &e0, &e1, &je, 2);
     if (r < n) {
         st.se[po0 + je] = e0;
         st.sv[po0 + r] = vnew[0][r];

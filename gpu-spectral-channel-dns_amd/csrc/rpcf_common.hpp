@@ -31,7 +31,8 @@ struct MemberParams {
     double aphi, beta, Re, dt, Ro;
     double q_re, q_im;
     int flow_control;
-    int parity;  // 1: even base profile, the solves read the parity blocks of the inverses
+    int parity;  // even base profile: 1, the solves read the parity blocks of the inverses;
+                 // 2, the parity blocks and their BF16 remainders (the full inverse's values)
 };
 
 // Member addressing of one grouped launch: launch member j runs slot ids[j]; a per-slot

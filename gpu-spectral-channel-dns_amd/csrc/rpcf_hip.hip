@@ -107,6 +107,7 @@ struct rpcf_ctx {
     double2* WL = nullptr;    // [5][P][nz] operator products of the linear right-hand sides
     double2* MINV = nullptr;  // [2][nslot][nz*nz] inverses, column-major
     double2* MINVH = nullptr; // [2][nslot][np][nz] parity blocks of the inverses (even base profiles)
+    unsigned* MINVE = nullptr; // [2][nslot][np][nz] their remainders, BF16 (re, im) (FP64 form)
     int4* PAIRS = nullptr;    // [2][np] row pairs (i, i', sign, 0) of the v and eta systems
     int npair = 0;            // np = N/2 + 1
     double* TF = nullptr;     // [5][RT][S][64] MFMA A-operand tiles of T0, T2, T4q, T0, T2q
@@ -116,7 +117,7 @@ struct rpcf_ctx {
     double* MEANU = nullptr;                             // [pz] base profile at the padded points
     MemberParams* MP = nullptr;                          // [capacity]
     int* PH = nullptr;                                   // [capacity] history phase
-    long sF = 0, sS = 0, sNL = 0, sY = 0, sW = 0, sZ = 0, sWl = 0, sM = 0, sMH = 0, sTf = 0, sQ = 0, sU = 0, sPz = 0;
+    long sF = 0, sS = 0, sNL = 0, sY = 0, sW = 0, sZ = 0, sWl = 0, sM = 0, sMH = 0, sME = 0, sTf = 0, sQ = 0, sU = 0, sPz = 0;
     // shape tables shared by all slots
     double *T0 = nullptr, *T2 = nullptr, *T4 = nullptr, *T0t = nullptr, *T2t = nullptr, *qw = nullptr, *tw = nullptr;
     std::vector<double> T0h, T2h, T4h, qw_host;
@@ -180,15 +181,17 @@ int check_sizes(const rpcf_params* prm, Dims* out) {
 
 // per-slot element counts of every per-slot array (= the slot strides)
 struct SlotSizes {
-    long F, S, NL, Y, W, Z, Wl, M, MH, Tf, Q, U, Pz;
+    long F, S, NL, Y, W, Z, Wl, M, MH, ME, Tf, Q, U, Pz;
     size_t bytes() const {
-        return sizeof(double2) * (size_t)(F + 2 * S + NL + Y + W + Z + Wl + M + MH) +
+        return sizeof(double2) * (size_t)(F + 2 * S + NL + Y + W + Z + Wl + M + MH) + sizeof(unsigned) * (size_t)ME +
                sizeof(double) * (size_t)(Tf + 2 * Q + 3 * U + Pz) + sizeof(MemberParams) + 2 * sizeof(int);
     }
 };
 
-// parity blocks: kept for the mixed form only (the FP64 form solves with the full inverses,
-// whose rounding the reference shares; the blocks agree with them to ~1e-11 after a step)
+// Inverses: the full ones (the mean flow's slot, odd base profiles, and the source of the
+// rest), their parity blocks (even base profiles), and in the FP64 form the blocks' BF16
+// remainders, which restore the full inverses' values (the blocks alone agree with them to
+// ~1e-11 after a step, enough for the mixed form only).
 SlotSizes slot_sizes(const Dims& d, int precision) {
     SlotSizes s;
     const long P = d.P, nz = d.nz, zld = (d.hnx + 7) & ~7;
@@ -201,7 +204,8 @@ SlotSizes slot_sizes(const Dims& d, int precision) {
     s.Z = 3L * d.pz * d.my * zld;
     s.Wl = 5 * P * nz;
     s.M = 2L * d.nslot * nz * nz;
-    s.MH = precision ? 2L * d.nslot * nz * (d.N / 2 + 1) : 0;  // nz rows x (N/2 + 1) pair columns
+    s.MH = 2L * d.nslot * nz * (d.N / 2 + 1);  // nz rows x (N/2 + 1) pair columns
+    s.ME = precision == 0 ? s.MH : 0;
     s.Tf = 5 * 2 * RT * S * 64;
     s.Q = nz * nz;
     s.U = nz;
@@ -712,7 +716,7 @@ int enq_lin_solve(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
             // about four column groups of nz threads (all of kSolveThreads at nz = 129)
             const int thr = std::min(kSolveThreads, 64 * ((4 * d.nz + 63) / 64));
             k_lin_solve<<<dim3(d.nslot, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
-                                                                 c->MINVH, c->sMH, c->PAIRS, c->npair,
+                                                                 c->MINVH, c->sMH, c->MINVE, c->sME, c->PAIRS, c->npair,
                                                                  c->WL, c->sWl, d, members(c, ids), c->T0t, c->U, c->DU,
                                                                  c->DDU, c->sU, c->qw);
         },
@@ -859,7 +863,7 @@ void free_ctx(rpcf_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     void* ptrs[] = {c->F,   c->SV, c->SE, c->NL,  c->Y,     c->W,  c->Z2, c->WL, c->MINV, c->TF,  c->T4Q,
                     c->T2Q, c->U,  c->DU, c->DDU, c->MEANU, c->MP, c->PH, c->T0, c->T2,   c->T4,  c->T0t,
-                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->PAIRS, c->MT, c->DT};
+                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->MINVE, c->PAIRS, c->MT, c->DT};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -932,7 +936,7 @@ int init_ctx(rpcf_ctx* c) {
 
     const SlotSizes ss = slot_sizes(d, c->shape.precision);
     c->sF = ss.F; c->sS = ss.S; c->sNL = ss.NL; c->sY = ss.Y; c->sW = ss.W; c->sZ = ss.Z;
-    c->sWl = ss.Wl; c->sM = ss.M; c->sMH = ss.MH; c->sTf = ss.Tf; c->sQ = ss.Q; c->sU = ss.U; c->sPz = ss.Pz;
+    c->sWl = ss.Wl; c->sM = ss.M; c->sMH = ss.MH; c->sME = ss.ME; c->sTf = ss.Tf; c->sQ = ss.Q; c->sU = ss.U; c->sPz = ss.Pz;
     c->zld = (d.hnx + 7) & ~7;
     c->tv_rt = (d.N / 2 + 16) / 16;
     c->tv_s = (d.N / 2 + 4) / 4;
@@ -940,6 +944,7 @@ int init_ctx(rpcf_ctx* c) {
         (rc = dalloc(c, &c->NL, C * ss.NL)) || (rc = dalloc(c, &c->Y, C * ss.Y)) || (rc = dalloc(c, &c->W, C * ss.W)) ||
         (rc = dalloc(c, &c->Z2, C * ss.Z)) || (rc = dalloc(c, &c->WL, C * ss.Wl)) ||
         (rc = dalloc(c, &c->MINV, C * ss.M)) || (rc = dalloc(c, &c->MINVH, C * ss.MH)) ||
+        (ss.ME && (rc = dalloc(c, &c->MINVE, C * ss.ME))) ||
         (rc = dalloc(c, &c->TF, C * ss.Tf)) ||
         (rc = dalloc(c, &c->T4Q, C * ss.Q)) || (rc = dalloc(c, &c->T2Q, C * ss.Q)) ||
         (rc = dalloc(c, &c->U, C * ss.U)) || (rc = dalloc(c, &c->DU, C * ss.U)) || (rc = dalloc(c, &c->DDU, C * ss.U)) ||
@@ -1153,8 +1158,9 @@ int init_ctx(rpcf_ctx* c) {
 // parity blocks of a slot's inverses (k_parity_blocks), on the context stream
 int build_parity_blocks(rpcf_ctx* c, int slot) {
     const Dims d = c->d;
-    k_parity_blocks<<<dim3(d.nslot, 2), 256, 0, c->stream>>>(c->MINV + slot * c->sM, c->MINVH + slot * c->sMH, c->PAIRS,
-                                                             c->npair, d);
+    k_parity_blocks<<<dim3(d.nslot, 2), 256, 0, c->stream>>>(c->MINV + slot * c->sM, c->MINVH + slot * c->sMH,
+                                                             c->MINVE ? c->MINVE + slot * c->sME : nullptr,
+                                                             c->PAIRS, c->npair, d);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? RPCF_OK : fail(c, RPCF_EDEVICE, "k_parity_blocks: %s", hipGetErrorString(e));
 }
@@ -1180,8 +1186,12 @@ int init_slot(rpcf_ctx* c, int slot, const rpcf_params* prm) {
     sh.mp.q_re = sh.mp.q_im = 0.0;
     sh.mp.flow_control = prm->flow_control;
     // an even base profile U(y) (no linear part) keeps the collocation systems symmetric under
-    // y -> -y, so the solves of the mixed form use the parity blocks of the inverses
-    sh.mp.parity = c->shape.precision != 0 && prm->u_profile[1] == 0.0 ? 1 : 0;
+    // y -> -y, so the solves use the parity blocks of the inverses: with their remainders in
+    // the FP64 form (the full inverses' values), alone in the others.  RPCF_FULL_INVERSE=1
+    // keeps the full inverses.
+    const char* fe = getenv("RPCF_FULL_INVERSE");
+    const bool full = fe && atoi(fe) != 0;
+    sh.mp.parity = full || prm->u_profile[1] != 0.0 ? 0 : c->shape.precision == 0 ? 2 : 1;
 
     std::vector<double> U(nz), dU(nz), ddU(nz), mU(d.pz);
     rpcf_host::base_profile(d.nz, prm->u_profile, U.data(), dU.data(), ddU.data());
@@ -1906,6 +1916,7 @@ int rpcf_member_adopt(rpcf_ctx* c, int slot, rpcf_ctx* src, int src_slot) {
     cp(c->NL, src->NL, c->sNL);
     cp(c->MINV, src->MINV, c->sM);
     cp(c->MINVH, src->MINVH, c->sMH);
+    cp(c->MINVE, src->MINVE, c->sME);
     cp(c->TF, src->TF, c->sTf);
     cp(c->T4Q, src->T4Q, c->sQ);
     cp(c->T2Q, src->T2Q, c->sQ);

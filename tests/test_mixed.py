@@ -122,6 +122,35 @@ def test_group_precision_must_match():
             g.init(0, q)
 
 
+@pytest.mark.parametrize("name", ["C1", "DEEP"])
+def test_fp64_solve_remainders_restore_full_inverse(name, monkeypatch):
+    """precision 0 with an even base profile solves with the parity blocks and their BF16
+    remainders (5/8 of the full inverses' bytes), which restore the full inverses' values:
+    within 2e-13 of the full-inverse state (RPCF_FULL_INVERSE=1) after one step and 1e-11
+    after 20, and at least 50 times closer to it than the blocks alone (precision 2) after
+    one step: the BF16 rounding (2^-9 relative) of the remainders leaves ~2^-9 of the gap
+    the blocks alone open (1.4e-11 at 128x129x128), 7e-14 measured.  (The numpy
+    reconstruction of one solve on the oracle's 128x129x128 inverses measured 3e-16
+    relative with exact remainders, <= 2.5e-15 with BF16 ones and up to 7.6e-13 without
+    them.)"""
+    prm, ic, _ = CASES[name]
+    a, _ = run(fresh(prm), ic, 1)
+    a20, _ = run(fresh(prm), ic, 20)
+    monkeypatch.setenv("RPCF_FULL_INVERSE", "1")
+    f, _ = run(fresh(prm), ic, 1)
+    f20, _ = run(fresh(prm), ic, 20)
+    monkeypatch.delenv("RPCF_FULL_INVERSE")
+    q = fresh(prm)
+    q.precision = 2
+    b, _ = run(q, ic, 1)
+    e1 = max(rel(a[i], f[i]) for i in (0, 1))
+    e20 = max(rel(a20[i], f20[i]) for i in (0, 1))
+    eb = max(rel(b[i], f[i]) for i in (0, 1))
+    print("%s: remainders vs full inverse %.2e (1 step) %.2e (20 steps); blocks alone %.2e" % (name, e1, e20, eb))
+    assert e1 < 2e-13 and e20 < 1e-11
+    assert eb > 50 * e1
+
+
 def test_fp64_parity_solves():
     """precision 2: FP64 with the reflection-parity blocks of the inverses for an even base
     profile -- within 1e-10 of the full-inverse FP64 state after one step and 1e-8 after 20
