@@ -46,17 +46,23 @@ typedef struct rpcf_ctx rpcf_ctx;
  *                 source/nonlinear.cu:400-403; {0.5,0.5,0} is the reference.
  *   flow_control  0: none (reference); 1: constant flow rate -- a uniform mean
  *                 pressure-gradient correction keeps the bulk of u0 at its value at set_state.
- *   precision     0: FP64 everywhere (reference REAL_DOUBLE, include/data.h:11);
- *                 1: mixed (HIP library, compiled FFT lengths only) -- the transform
- *                 intermediates, the x and span FFTs and the Lamb vector in FP32; the
- *                 Chebyshev transforms, the nonlinear combination, the operators and the
- *                 solves in FP64, the solves of an even base profile (u_profile[1] == 0)
- *                 with the reflection-parity blocks of the inverses (half their bytes;
- *                 they agree with the full inverses to round-off of the inverse).
- *                 2: FP64 with those parity-block solves (even base profiles; an odd
- *                 profile runs as 0): the state agrees with precision 0 and the reference
- *                 to about 1e-11 after one step instead of 1e-15.  The oracle computes FP64
- *                 with full inverses whatever this field says. */
+ *   precision     0: FP64 everywhere (reference REAL_DOUBLE, include/data.h:11); the HIP
+ *                 library solves an even base profile (u_profile[1] == 0) with the
+ *                 reflection-parity blocks of the inverses (half their bytes) plus the
+ *                 blocks' remainders in BF16 (an eighth), which restore the full inverses'
+ *                 values: the state agrees with the full-inverse solve to ~1e-13 after one
+ *                 step (environment RPCF_FULL_INVERSE=1 keeps the full inverses).
+ *                 1: mixed (HIP library; the matrix-core form for 2(nz-1) <= 384) -- the
+ *                 transform intermediates, the x and span FFTs and the Lamb vector in FP32,
+ *                 the wall-normal Chebyshev transforms as split-FP16 matrix-core products
+ *                 with FP32 accumulation; velocity recovery, the nonlinear terms, the
+ *                 operators and the solves in FP64, an even base profile's solves with
+ *                 the parity blocks alone (they agree with the full inverses to the
+ *                 inverse's own round-off asymmetry).
+ *                 2: FP64 with the parity blocks alone (even base profiles; an odd profile
+ *                 runs as 0): the state agrees with precision 0 and the reference to about
+ *                 1e-11 after one step.  The oracle computes FP64 with full inverses
+ *                 whatever this field says. */
 typedef struct rpcf_params {
     int mx, my, mz;
     double n_pi_x, n_pi_y;
