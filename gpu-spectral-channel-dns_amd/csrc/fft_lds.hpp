@@ -30,7 +30,8 @@ struct FftPlan {
     int n;
     int nst;
     int rad[16];
-    const void* tw;  // n twiddles exp(-2 pi i k/n), double2 or float2
+    const void* tw;  // n twiddles exp(-2 pi i k/n), double2 or float2 (then, for the
+                     // two-pass lengths, the stage-1 table: ct_tw1_count below)
 };
 
 template <typename T> struct cplx_t;
@@ -303,6 +304,27 @@ constexpr int ct2_rad(int n, int s) {
          : n == 384 ? (s == 0 ? 16 : (s == 1 ? 12 : 2))
          : 0;
 }
+// Stage 1 of a two-pass plan multiplies input r of butterfly k (k < NS, NS the first radix)
+// by w^(k r tstep), tstep = n/(NS R).  Read as tw[k r tstep], the lanes k = 0, 1, .. of a
+// wave hit LDS at stride r: 2-, 4- or 8-way bank conflicts for even r.  The plan's twiddle
+// table therefore carries, behind its n entries exp(-2 pi i m/n), the same values laid out
+// [r - 1][k] (entry i: m = ct_tw1_index(n, i)), read at consecutive addresses.
+constexpr int ct_tw1_count(int n) { return ct2_nst(n) >= 2 ? (ct2_rad(n, 1) - 1) * ct2_rad(n, 0) : 0; }
+constexpr int ct_tw_len(int n) { return n + ct_tw1_count(n); }
+constexpr int ct_tw1_index(int n, int i) {
+    return (i % ct2_rad(n, 0)) * (i / ct2_rad(n, 0) + 1) * (n / (ct2_rad(n, 0) * ct2_rad(n, 1)));
+}
+// The twiddles a compiled plan reads, as kernels copy them into LDS: the plain entries
+// m < ct_tw_plain(n) its later stages read (none for a two-stage plan, whose stage 0 has no
+// twiddles; m < n/2 for the last, radix-2 stage of the three-stage one; all n for the
+// radix-8 plans), then the stage-1 table -- fewer entries than n for the two-pass plans.
+constexpr int ct_tw_plain(int n) { return ct2_nst(n) == 2 ? 0 : ct2_nst(n) == 3 ? n / ct2_rad(n, 2) : n; }
+constexpr int ct_tw_lds_len(int n) { return ct_tw_plain(n) + ct_tw1_count(n); }
+// the plan-table entry that entry i of the LDS copy holds, and its twiddle index m
+constexpr int ct_tw_lds_src(int n, int i) { return i < ct_tw_plain(n) ? i : n + (i - ct_tw_plain(n)); }
+constexpr int ct_tw_lds_index(int n, int i) {
+    return i < ct_tw_plain(n) ? i : ct_tw1_index(n, i - ct_tw_plain(n));
+}
 constexpr int ct_plan_nst(int n) { return ct2_nst(n) ? ct2_nst(n) : ct_nst(n); }
 constexpr int ct_plan_rad(int n, int s) { return ct2_nst(n) ? ct2_rad(n, s) : ct_rad(n, s); }
 
@@ -316,7 +338,7 @@ template <int Q> __device__ __forceinline__ unsigned swz(unsigned a) {
     else return a ^ ((a / Q) % Q);
 }
 
-template <int R, int DIR, int N, int NS, int BT, typename C, int QW = 0, int QR = 0>
+template <int R, int DIR, int N, int NS, int BT, typename C, int QW = 0, int QR = 0, bool TW1 = false, int T1OFF = N>
 __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __restrict__ tw, int tid, int nthr) {
     // unsigned index arithmetic: divisions by the compile-time strides become shifts and
     // multiplies without sign fix-ups
@@ -335,7 +357,7 @@ __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __r
             for (int r = 0; r < R; r++) {
                 C x = base[swz<QR>(j + r * nR)];
                 if (r > 0 && NS > 1) {
-                    C w = tw[k * r * tstep];
+                    C w = TW1 ? tw[T1OFF + (r - 1) * NS + k] : tw[k * r * tstep];
                     if (DIR > 0) w.y = -w.y;
                     x = cmul(x, w);
                 }
@@ -359,25 +381,29 @@ __device__ __forceinline__ void fft_stage_ct(C* buf, int ld, int B, const C* __r
     __syncthreads();
 }
 
-// TWO = 1: the two-pass plan where one exists; TWO = 0: the radix-8 plan (fewer registers)
-template <int N, int DIR, int BT, int TWO, typename C, int S = 0, int NS = 1>
+// TWO = 1: the two-pass plan where one exists; TWO = 0: the radix-8 plan (fewer registers).
+// T1OFF: the first stage-1 twiddle in tw (N in a plan table, ct_tw_plain(N) in an LDS copy).
+template <int N, int DIR, int BT, int TWO, typename C, int T1OFF = N, int S = 0, int NS = 1>
 __device__ __forceinline__ void fft_ct(C* buf, int ld, int B, const C* tw, int tid, int nthr) {
     constexpr int nst = TWO ? ct_plan_nst(N) : ct_nst(N);
     if constexpr (S < nst) {
         constexpr int R = TWO ? ct_plan_rad(N, S) : ct_rad(N, S);
         constexpr int R0 = TWO ? ct_plan_rad(N, 0) : ct_rad(N, 0);
         constexpr int Q = (nst >= 2 && (R0 == 8 || R0 == 16)) ? R0 : 0;  // swizzle between stages 0 and 1
-        fft_stage_ct<R, DIR, N, NS, BT, C, S == 0 ? Q : 0, S == 1 ? Q : 0>(buf, ld, B, tw, tid, nthr);
-        fft_ct<N, DIR, BT, TWO, C, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
+        constexpr bool TW1 = TWO && S == 1 && ct2_nst(N) >= 2;  // stage 1 from the [r-1][k] table
+        fft_stage_ct<R, DIR, N, NS, BT, C, S == 0 ? Q : 0, S == 1 ? Q : 0, TW1, T1OFF>(buf, ld, B, tw, tid, nthr);
+        fft_ct<N, DIR, BT, TWO, C, T1OFF, S + 1, NS * R>(buf, ld, B, tw, tid, nthr);
     }
 }
 
 // FN > 0: the compile-time plan of length FN (must equal pl.n); FN = 0: the runtime plan.
-// BT > 0 (compiled plans only): B <= BT sequences and nthr == kFftBlock.
-template <int FN, int DIR, int BT = 0, int TWO = 1, typename C>
+// BT > 0 (compiled plans only): B <= BT sequences and nthr == kFftBlock.  LDSTW: pl.tw is
+// the LDS copy of the plan's twiddles (ct_tw_lds_len entries, ct_tw_lds_src).
+template <int FN, int DIR, int BT = 0, int TWO = 1, bool LDSTW = false, typename C>
 __device__ __forceinline__ void fft_any(C* buf, int ld, int B, const FftPlan& pl, int tid, int nthr) {
     static_assert(FN == 0 || BT > 0, "compiled plans need the tile bound (their radix-12/16 stages size registers by it)");
-    if constexpr (FN > 0) fft_ct<FN, DIR, BT, TWO>(buf, ld, B, (const C*)pl.tw, tid, nthr);
+    if constexpr (FN > 0)
+        fft_ct<FN, DIR, BT, TWO, C, LDSTW ? ct_tw_plain(FN) : FN>(buf, ld, B, (const C*)pl.tw, tid, nthr);
     else fft_run<DIR>(buf, ld, B, pl, tid, nthr);
 }
 

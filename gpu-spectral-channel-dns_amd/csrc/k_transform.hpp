@@ -180,7 +180,8 @@ __global__ __launch_bounds__(256) void k_cheb_s2p(const double2* __restrict__ F,
 
 // LDS elements (double2) of k_cheb_s2p_h for a tile of tb pencils
 __host__ __device__ constexpr size_t s2p_h_lds(int fn, int tb) {
-    return 2 * (size_t)tb * (fn / 2 + 2) + (size_t)tb * (fn / 4 + 2) + fn + fn / 4;
+    return 2 * (size_t)tb * (fn / 2 + 2) + (size_t)tb * (fn / 4 + 2) + ct_tw_lds_len(fn / 2) + ct_tw_lds_len(fn / 4) +
+           fn / 2;
 }
 
 // -------------------------------------------------------------------------------------
@@ -218,14 +219,12 @@ __global__ __launch_bounds__(256) void k_cheb_s2p_h(const double2* __restrict__ 
     const int nb = NB ? NB : nb_rt;
     double2* A = lds;
     double2* B = A + 2 * TB * LA;
-    double2* twN = B + TB * LB;
-    double2* twH = twN + N;
-    double2* pre = twH + H;  // tw4[k], k < N (conjugated where used)
-    for (int i = tid; i < N; i += nthr) {
-        twN[i] = tw4[4 * i];
-        pre[i] = tw4[i];
-    }
-    for (int i = tid; i < H; i += nthr) twH[i] = tw4[8 * i];
+    double2* twN = B + TB * LB;  // the LDS twiddle copies of the length N and N/2 plans
+    double2* twH = twN + ct_tw_lds_len(N);
+    double2* pre = twH + ct_tw_lds_len(H);  // tw4[k], k < N (conjugated where used)
+    for (int i = tid; i < N; i += nthr) pre[i] = tw4[i];
+    for (int i = tid; i < ct_tw_lds_len(N); i += nthr) twN[i] = tw4[4 * ct_tw_lds_index(N, i)];
+    for (int i = tid; i < ct_tw_lds_len(H); i += nthr) twH[i] = tw4[8 * ct_tw_lds_index(H, i)];
     const int pstride = d.hnx * nz;
     const double2* in = F + (f * fstride + (ky0 * d.hnx + kx) * nz);
     double2* stg = A + TB * LA;
@@ -296,8 +295,8 @@ __global__ __launch_bounds__(256) void k_cheb_s2p_h(const double2* __restrict__ 
     FftPlan pn{}, ph{};
     pn.tw = twN;
     ph.tw = twH;
-    fft_any<N, +1, 2 * TBX>(A, LA, 2 * nb, pn, tid, nthr);
-    fft_any<H, +1, TBX>(B, LB, nb, ph, tid, nthr);
+    fft_any<N, +1, 2 * TBX, 1, true>(A, LA, 2 * nb, pn, tid, nthr);
+    fft_any<H, +1, TBX, 1, true>(B, LB, nb, ph, tid, nthr);
     cplx<RO>* out = Y + (f * ystride + kx * pz * ny + ky0);
     staged<8>(
         pz * nb, tid, nthr,
@@ -362,11 +361,11 @@ __global__ __launch_bounds__(256) void k_span_inv(const cplx<R>* __restrict__ Y,
         FftPlan pll = pl;  // compiled lengths: twiddles from an LDS copy behind the tile
         if constexpr (FN > 0) {
             C2* twl = lds + TK * ldl;
-            for (int i = tid; i < FN; i += nthr) twl[i] = ((const C2*)pl.tw)[i];
+            for (int i = tid; i < ct_tw_lds_len(FN); i += nthr) twl[i] = ((const C2*)pl.tw)[ct_tw_lds_src(FN, i)];
             pll.tw = twl;
         }
         __syncthreads();
-        fft_any<FN, +1, span_tile_max(FN)>(lds, ldl, nk, pll, tid, nthr);
+        fft_any<FN, +1, span_tile_max(FN), 1, true>(lds, ldl, nk, pll, tid, nthr);
         C2* out = W + f * wstride;
         staged<12>(
             my * nk, tid, nthr,
@@ -488,11 +487,11 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
         if constexpr (FN > 0) {
             C2* twl = lds + 6 * RP * lds_ld;
             const C2* twg = (const C2*)pl.tw;
-            for (int i = tid; i < FN; i += nthr) twl[i] = twg[i];
+            for (int i = tid; i < ct_tw_lds_len(FN); i += nthr) twl[i] = twg[ct_tw_lds_src(FN, i)];
             pll.tw = twl;
         }
         __syncthreads();
-        fft_any<FN, +1, 6 * lamb_pairs_max(FN)>(lds, lds_ld, 6 * nrp, pll, tid, nthr);
+        fft_any<FN, +1, 6 * lamb_pairs_max(FN), 1, true>(lds, lds_ld, 6 * nrp, pll, tid, nthr);
         // Ro U(y) is uniform in x and span, so it reaches only the (0,0) pencil of L_z, which
         // rhsNonlinear does not use: the FP32 form leaves it out (in FP32 its round-off would
         // swamp the perturbation products); the FP64 form keeps the reference's expression.
@@ -510,7 +509,7 @@ __global__ __launch_bounds__(256) void k_lamb_phys(const cplx<R>* __restrict__ W
             lds[(2 * nrp + ip) * lds_ld + x] = cmake<C2>(lza, lzb);
         }
         __syncthreads();
-        fft_any<FN, -1, 3 * lamb_pairs_max(FN)>(lds, lds_ld, 3 * nrp, pll, tid, nthr);
+        fft_any<FN, -1, 3 * lamb_pairs_max(FN), 1, true>(lds, lds_ld, 3 * nrp, pll, tid, nthr);
         for (int i = tid; i < 3 * nrp * hnx; i += nthr) {
             const int seq = i / hnx, k = i - seq * hnx;
             const int g = seq / nrp, ip = seq - g * nrp;
@@ -614,11 +613,11 @@ __global__ __launch_bounds__(256) void k_span_fwd(const cplx<R>* __restrict__ Z2
         FftPlan pll = pl;  // compiled lengths: twiddles from an LDS copy behind the tile
         if constexpr (FN > 0) {
             C2* twl = lds + TK * ldl;
-            for (int i = tid; i < FN; i += nthr) twl[i] = ((const C2*)pl.tw)[i];
+            for (int i = tid; i < ct_tw_lds_len(FN); i += nthr) twl[i] = ((const C2*)pl.tw)[ct_tw_lds_src(FN, i)];
             pll.tw = twl;
         }
         __syncthreads();
-        fft_any<FN, -1, span_tile_max(FN)>(lds, ldl, nk, pll, tid, nthr);
+        fft_any<FN, -1, span_tile_max(FN), 1, true>(lds, ldl, nk, pll, tid, nthr);
         C2* out = Z1 + g * ostride;
         staged<8>(
             nk * ny, tid, nthr,
@@ -1041,10 +1040,10 @@ __global__ __launch_bounds__(256, p2s_waves(FN)) void k_p2s_nl2(const cplx<RI>* 
     double2* fb = lds;
     double2* tot = lds + 2 * TB * ld;
     double2* twl = tot + 256;
-    double2* tw4l = twl + FN;
+    double2* tw4l = twl + ct_tw_lds_len(FN);
     {
         const double2* twg = (const double2*)plN.tw;
-        for (int i = tid; i < FN; i += nthr) twl[i] = twg[i];
+        for (int i = tid; i < ct_tw_lds_len(FN); i += nthr) twl[i] = twg[ct_tw_lds_src(FN, i)];
         for (int i = tid; i < nz; i += nthr) tw4l[i] = tw4[i];
     }
     FftPlan pll = plN;
@@ -1090,7 +1089,7 @@ __global__ __launch_bounds__(256, p2s_waves(FN)) void k_p2s_nl2(const cplx<RI>* 
         }
         if constexpr (g < 2) load(g + 1);
         __syncthreads();
-        fft_any<FN, -1, 2 * TBC>(fb, ld, 2 * nb, pll, tid, nthr);
+        fft_any<FN, -1, 2 * TBC, 1, true>(fb, ld, 2 * nb, pll, tid, nthr);
 #pragma unroll
         for (int u = 0; u < KC; u++) {
             const int i = tid + u * nthr;
@@ -1160,7 +1159,7 @@ __global__ __launch_bounds__(256, p2s_waves(FN)) void k_p2s_nl2(const cplx<RI>* 
         }
     }
     __syncthreads();
-    fft_any<FN, -1, 2 * TBC>(fb, ld, 2 * nb, pll, tid, nthr);
+    fft_any<FN, -1, 2 * TBC, 1, true>(fb, ld, 2 * nb, pll, tid, nthr);
     staged<8>(
         2 * total, tid, nthr,
         [&](int i) {

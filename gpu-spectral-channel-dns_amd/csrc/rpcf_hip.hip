@@ -303,13 +303,25 @@ int zero_sync(rpcf_ctx* c, void* dst, size_t bytes) {
     return RPCF_OK;
 }
 
+// a plan's twiddle table: exp(-2 pi i m/n), m < n, then (two-pass lengths) the stage-1
+// table, the same values laid out [r - 1][k] (ct_tw1_count, fft_lds.hpp); 2 ct_tw_len(n) doubles
+std::vector<double> plan_twiddles(int n) {
+    std::vector<double> tw(2 * (size_t)ct_tw_len(n));
+    rpcf_host::twiddles(n, tw.data());
+    for (int i = 0; i < ct_tw1_count(n); i++) {
+        const int m = ct_tw1_index(n, i);
+        tw[2 * (n + i)] = tw[2 * m];
+        tw[2 * (n + i) + 1] = tw[2 * m + 1];
+    }
+    return tw;
+}
+
 int make_plan(rpcf_ctx* c, int n, double* tw_dev, FftPlan* pl, int* cap) {
     pl->n = n;
     if (!rpcf_host::factorize(n, pl->rad, &pl->nst)) return fail(c, RPCF_EINVAL, "FFT length %d not 2^a3^b5^c", n);
     *cap = fft_capacity(pl->rad, pl->nst, kFftThreads);
-    std::vector<double> tw(2 * (size_t)n);
-    rpcf_host::twiddles(n, tw.data());
-    RCCHK(copy_sync(c, tw_dev, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    std::vector<double> tw = plan_twiddles(n);
+    RCCHK(copy_sync(c, tw_dev, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
     pl->tw = tw_dev;
     return RPCF_OK;
 }
@@ -519,7 +531,7 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     {
         const int TK = c->TK_inv, ld = c->ld_my;
         dim3 g((d.nxh + TK - 1) / TK, d.pz, 6 * n);
-        size_t lds = sizeof(C2) * (TK * ld + (compiled_len(d.my) ? d.my : 0));
+        size_t lds = sizeof(C2) * (TK * ld + (compiled_len(d.my) ? ct_tw_lds_len(d.my) : 0));
         rc = launch(
             c, KID_SPAN_INV,
             [&] {
@@ -539,7 +551,7 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     {
         const int RP = c->RP, ld = c->ld_mx;
         dim3 g((d.my / 2 + RP - 1) / RP, d.pz, n);
-        size_t lds = sizeof(C2) * (6 * RP * ld + (compiled_len(d.mx) ? d.mx : 0));
+        size_t lds = sizeof(C2) * (6 * RP * ld + (compiled_len(d.mx) ? ct_tw_lds_len(d.mx) : 0));
         rc = launch(
             c, KID_LAMB,
             [&] {
@@ -560,7 +572,7 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     {
         const int TK = c->TK_fwd, ld = c->ld_my;
         dim3 g((d.hnx + TK - 1) / TK, d.pz, 3 * n);
-        size_t lds = sizeof(C2) * (TK * ld + (compiled_len(d.my) ? d.my : 0));
+        size_t lds = sizeof(C2) * (TK * ld + (compiled_len(d.my) ? ct_tw_lds_len(d.my) : 0));
         rc = launch(
             c, KID_SPAN_FWD,
             [&] {
@@ -625,7 +637,7 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     if (c->TB_nl2) {
         const int TB = c->TB_nl2, ld = c->ld_zn;
         dim3 g((d.ny + TB - 1) / TB, d.hnx, n);
-        const size_t lds = sizeof(double2) * (2 * (size_t)TB * ld + 256 + 2 * d.N + d.nz);
+        const size_t lds = sizeof(double2) * (2 * (size_t)TB * ld + 256 + ct_tw_lds_len(2 * d.N) + d.nz);
         return launch(
             c, KID_P2S_NL,
             [&] {
@@ -956,7 +968,9 @@ int init_ctx(rpcf_ctx* c) {
     if ((rc = dalloc(c, &c->T0, nz * nz)) || (rc = dalloc(c, &c->T2, nz * nz)) || (rc = dalloc(c, &c->T4, nz * nz)) ||
         (rc = dalloc(c, &c->T0t, nz * nz)) || (rc = dalloc(c, &c->T2t, nz * nz)) || (rc = dalloc(c, &c->qw, nz)))
         return rc;
-    if ((rc = dalloc(c, &c->tw, 2 * ((size_t)d.mx + d.my + 4 * d.N + 2 * d.N)))) return rc;
+    if ((rc = dalloc(c, &c->tw, 2 * ((size_t)ct_tw_len(d.mx) + ct_tw_len(d.my) + ct_tw_len(4 * d.N) +
+                                       ct_tw_len(2 * d.N)))))
+        return rc;
     RCCHK(zero_sync(c, c->ids_zero, sizeof(int)));
     RCCHK(zero_sync(c, c->PH, sizeof(int) * C));
     {
@@ -1006,15 +1020,13 @@ int init_ctx(rpcf_ctx* c) {
     // FFT plans (the reference's cuFFT plans, source/transform.cu:49-71)
     double* tw = c->tw;
     if ((rc = make_plan(c, d.mx, tw, &c->pl_x, &c->cap_x))) return rc;
-    tw += 2 * d.mx;
+    tw += 2 * ct_tw_len(d.mx);
     if ((rc = make_plan(c, d.my, tw, &c->pl_y, &c->cap_y))) return rc;
-    tw += 2 * d.my;
+    tw += 2 * ct_tw_len(d.my);
     if (c->mixed) {
         // FP32 copies of the x and span twiddles
-        std::vector<double> t64(2 * (size_t)d.mx), t64y(2 * (size_t)d.my);
-        rpcf_host::twiddles(d.mx, t64.data());
-        rpcf_host::twiddles(d.my, t64y.data());
-        std::vector<float> t32(2 * ((size_t)d.mx + d.my));
+        const std::vector<double> t64 = plan_twiddles(d.mx), t64y = plan_twiddles(d.my);
+        std::vector<float> t32(t64.size() + t64y.size());
         for (size_t i = 0; i < t64.size(); i++) t32[i] = (float)t64[i];
         for (size_t i = 0; i < t64y.size(); i++) t32[t64.size() + i] = (float)t64y[i];
         if ((rc = dalloc(c, &c->twf, t32.size()))) return rc;
@@ -1022,10 +1034,10 @@ int init_ctx(rpcf_ctx* c) {
         c->pl_xf = c->pl_x;
         c->pl_xf.tw = c->twf;
         c->pl_yf = c->pl_y;
-        c->pl_yf.tw = c->twf + 2 * d.mx;
+        c->pl_yf.tw = c->twf + t64.size();
     }
     if ((rc = make_plan(c, 4 * d.N, tw, &c->pl_zp, &c->cap_zp))) return rc;
-    tw += 2 * 4 * d.N;
+    tw += 2 * ct_tw_len(4 * d.N);
     if ((rc = make_plan(c, 2 * d.N, tw, &c->pl_zn, &c->cap_zn))) return rc;
     // mixed form: fragments of the split-FP16 Chebyshev products (k_cheb_mfma.hpp) for the
     // compiled lengths up to 2N = 384; RPCF_MIXED_MFMA=0 keeps the FFT-based kernels
@@ -1081,7 +1093,7 @@ int init_ctx(rpcf_ctx* c) {
 #define RPCF_CASE(L)                                                                                     \
     if (2 * d.N == L)                                                                                    \
         c->TB_nl2 = pick_tile(cheb_nl_tile_max(L), 1, cheb_nl_tile_max(L), lim / 2, [&](int t) {             \
-            return sizeof(double2) * (2 * (size_t)t * c->ld_zn + 256 + 2 * d.N + d.nz);                 \
+            return sizeof(double2) * (2 * (size_t)t * c->ld_zn + 256 + ct_tw_lds_len(2 * d.N) + d.nz);       \
         });
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
@@ -1101,11 +1113,11 @@ int init_ctx(rpcf_ctx* c) {
     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
     c->TK_inv = pick_tile(tk_max, d.my, c->cap_y, lim3, [&](int t) {
-        return sizeof(double2) * ((size_t)t * c->ld_my + (compiled_len(d.my) ? d.my : 0));
+        return sizeof(double2) * ((size_t)t * c->ld_my + (compiled_len(d.my) ? ct_tw_lds_len(d.my) : 0));
     });
     c->TK_fwd = c->TK_inv;
     c->RP = pick_tile(rp_max, 6 * d.mx, c->cap_x, lim3, [&](int t) {
-        return sizeof(double2) * (6 * (size_t)t * c->ld_mx + (compiled_len(d.mx) ? d.mx : 0));
+        return sizeof(double2) * (6 * (size_t)t * c->ld_mx + (compiled_len(d.mx) ? ct_tw_lds_len(d.mx) : 0));
     });
     if (!c->TB_s2p || !c->TB_nl || !c->TK_inv || !c->RP)
         return fail(c, RPCF_EINVAL, "grid too large for the LDS tiles (limit %d bytes)", lim);
