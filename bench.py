@@ -50,8 +50,8 @@ def kernel_bytes(s, prm):
     """algorithmic HBM bytes per launch of each kernel: the arrays it must read and write
     once.  FP64 form: complex FP64 (16 B) throughout; with an even base profile (the channel)
     the solves read the two parity blocks of each inverse (nz x (N/2 + 1) complex FP64
-    entries) and their remainders (the same count of complex BF16 pairs, 4 B), otherwise
-    the full inverses (nz x nz).  Mixed form (precision 1; channel profile, matrix-core
+    entries) and the remainders of the v blocks (the same count of complex BF16 pairs, 4 B),
+    otherwise the full inverses (nz x nz).  Mixed form (precision 1; channel profile, matrix-core
     Chebyshev transforms): the velocity and vorticity fields, the padded intermediates and
     the Lamb vector in complex FP32 (8 B), the state, nonlinear terms and operator outputs in
     FP64, and the solves with the parity blocks alone.  Precision 2: FP64 with the parity
@@ -67,9 +67,10 @@ def kernel_bytes(s, prm):
     Sp = hnx * ny * pz * 16 * h        # one field on the padded wall-normal points
     Wb = pz * my * nxh * 16 * h        # one field after the span transform (kx < nx/2)
     Zb = pz * my * hnx * 16 * h        # one Lamb component after the x transform
-    M1 = nz * nz * 16                  # one inverse matrix
-    if parity:  # its parity blocks: nz rows x (N/2 + 1) pair columns, in FP64 with BF16 remainders
-        M1 = nz * ((nz - 1) // 2 + 1) * (16 + (4 if prm.precision == 0 else 0))
+    np_ = (nz - 1) // 2 + 1            # parity pair columns
+    M2 = 2 * nz * nz * 16              # a slot's two inverses
+    if parity:  # their parity blocks: nz rows x (N/2 + 1) pair columns, FP64, and the v remainders
+        M2 = 2 * nz * np_ * 16 + (nz * np_ * 4 if prm.precision == 0 else 0)
     nslot = hnx * (ny // 2 + 1)        # matrix slots: pencils +-ky share one
     out = {
         "k_cheb_s2p": 6 * F + 6 * Sp,
@@ -78,7 +79,7 @@ def kernel_bytes(s, prm):
         "k_span_fwd": 3 * Zb + 3 * Sp,
         "k_p2s_nl": 3 * Sp + 2 * S,
         "k_lin_tv": 2 * S + 5 * S,
-        "k_lin_solve": 2 * nslot * M1 + 11 * S,
+        "k_lin_solve": nslot * M2 + 11 * S,
         "k_velocity": 2 * S + 6 * F,
     }
     return {k: int(v) for k, v in out.items()}

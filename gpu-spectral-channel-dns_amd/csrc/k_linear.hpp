@@ -267,7 +267,7 @@ struct LinState {
     double2* se;          // eta-hat (w0 in pencil 0)
     const double2* minv;  // [2][nslot][nz*nz] column-major inverses
     const double2* minvh; // [2][nslot][np][nz] their parity blocks (k_parity_blocks)
-    const unsigned* minve; // [2][nslot][np][nz] the blocks' remainders, BF16 (re, im) pairs
+    const unsigned* minve; // [nslot][np][nz] the v blocks' remainders, BF16 (re, im) pairs
     double qw_target_re, qw_target_im;  // flow-rate target (flow_control)
     int flow_control;
     int parity;           // 1: solve with the parity blocks, 2: with the blocks and remainders
@@ -728,9 +728,12 @@ __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const un
 // so  x_j = sum_p H[j][p] (b_i + (-1)^j s b_i') + E[j][p] (b_i - (-1)^j s b_i')  is M b
 // exactly.  E is the inverse's round-off asymmetry (~1e-14 of its largest entry at
 // 128x129x128, the difference of two nearly equal doubles, so computed exactly) and is
-// stored as BF16: its 2^-9 relative rounding is below the FP64 rounding of M itself, and
-// dropping E would move a solve by up to ~1e-12 (relative), so the blocks with their
-// remainders (5/8 of the bytes of M) keep the full inverse's results.  minve may be null.
+// stored as BF16: its 2^-9 relative rounding is below the FP64 rounding of M itself.
+// Dropping E moves a v solve by up to ~1e-12 (relative; its y-derivatives, which velocity
+// recovery takes, more), but an eta solve and its first two y-derivatives by <= 4e-15 (the
+// eta system is well conditioned), so only the v blocks keep remainders (same layout as H):
+// the solves read (2 x 16 + 4)/64 of the full inverses' bytes and keep their results.
+// minve (the v remainders) may be null.
 // -------------------------------------------------------------------------------------
 __device__ __forceinline__ unsigned bf16_bits(double x) {
     unsigned u = __float_as_uint((float)x);
@@ -742,7 +745,7 @@ __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __res
     const int slot = blockIdx.x, eq = blockIdx.y, n = d.nz, ne = d.N / 2 + 1;
     const double2* M = minv + ((long)eq * d.nslot + slot) * n * n;  // M[j][i] at i*n + j
     double2* Hm = minvh + ((long)eq * d.nslot + slot) * (long)n * np;
-    unsigned* Em = minve ? minve + ((long)eq * d.nslot + slot) * (long)n * np : nullptr;
+    unsigned* Em = minve && eq == 0 ? minve + (long)slot * n * np : nullptr;
     const int4* pr = pairs + eq * np;
     for (long e = threadIdx.x; e < (long)n * np; e += blockDim.x) {
         const int p = (int)(e / n), r = (int)(e - (long)p * n);
@@ -813,7 +816,8 @@ __device__ __forceinline__ void gemv_t0_reflect(const double* __restrict__ T0t, 
 // x = M b for the two right-hand sides rhs[0..1][nz] of a slot: with the full inverse M, or,
 // for a member with an even base profile, with its parity blocks H (k_parity_blocks): the
 // combined sides b_i + s b_i' and b_i - s b_i' per row pair, then one nz x np GEMV whose even
-// rows take the first and odd rows the second (parity 2: and the remainders E the other).  b vanishes on the rows below j0 (an even
+// rows take the first and odd rows the second (parity 2 and Em given: and the remainders E
+// the other).  b vanishes on the rows below j0 (an even
 // count: whole row pairs), whose columns are skipped.  Thread r < nz returns x at
 // coefficient *jrow.  Every thread of the block calls it (it synchronises).
 __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const unsigned* Em, const int4* pairs,
@@ -841,7 +845,7 @@ __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, 
     }
     __syncthreads();
     const int ne = N / 2 + 1;
-    if (parity == 2)
+    if (parity == 2 && Em)
         gemv_sel<true>(Hm, Em, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
     else
         gemv_sel<false>(Hm, Em, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
@@ -889,7 +893,7 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
         st.se = SE + ms * sS;
         st.minv = MINV + ms * sM;
         st.minvh = MINVH + ms * sMH;
-        st.minve = MINVE + ms * sME;
+        st.minve = MINVE ? MINVE + ms * sME : nullptr;
         st.parity = mp.parity;
         st.qw_target_re = mp.q_re;
         st.qw_target_im = mp.q_im;
@@ -988,8 +992,9 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
     {
         double2 y0, y1;
         int j;
-        solve_slot(Mv, st.minvh + (long)slot * nh, st.minve + (long)slot * nh, pairs, np, st.parity, rhs, bc,
-                   &part[0][0], n, d.N, r, blockDim.x, &y0, &y1, &j, 4);
+        const unsigned* Ev = st.minve ? st.minve + (long)slot * nh : nullptr;
+        solve_slot(Mv, st.minvh + (long)slot * nh, Ev, pairs, np, st.parity, rhs, bc, &part[0][0], n, d.N, r,
+                   blockDim.x, &y0, &y1, &j, 4);
         if (r < n) vnew[0][j] = y0, vnew[1][j] = y1;
     }
     __syncthreads();
@@ -1018,8 +1023,8 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
     __syncthreads();
     double2 e0, e1;
     int je;
-    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, st.minve + ((long)d.nslot + slot) * nh, pairs + np, np,
-               st.parity, rhs, bc, &part[0][0], n, d.N, r, blockDim.x, &e0, &e1, &je, 2);
+    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, nullptr, pairs + np, np, st.parity, rhs, bc, &part[0][0],
+               n, d.N, r, blockDim.x, &e0, &e1, &je, 2);  // eta: the blocks without remainders
     if (r < n) {
         st.se[po0 + je] = e0;
         st.sv[po0 + r] = vnew[0][r];

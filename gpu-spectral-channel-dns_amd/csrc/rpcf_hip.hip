@@ -107,7 +107,7 @@ struct rpcf_ctx {
     double2* WL = nullptr;    // [5][P][nz] operator products of the linear right-hand sides
     double2* MINV = nullptr;  // [2][nslot][nz*nz] inverses, column-major
     double2* MINVH = nullptr; // [2][nslot][np][nz] parity blocks of the inverses (even base profiles)
-    unsigned* MINVE = nullptr; // [2][nslot][np][nz] their remainders, BF16 (re, im) (FP64 form)
+    unsigned* MINVE = nullptr; // [nslot][np][nz] the v blocks' remainders, BF16 (re, im) (FP64 form)
     int4* PAIRS = nullptr;    // [2][np] row pairs (i, i', sign, 0) of the v and eta systems
     int npair = 0;            // np = N/2 + 1
     double* TF = nullptr;     // [5][RT][S][64] MFMA A-operand tiles of T0, T2, T4q, T0, T2q
@@ -189,9 +189,9 @@ struct SlotSizes {
 };
 
 // Inverses: the full ones (the mean flow's slot, odd base profiles, and the source of the
-// rest), their parity blocks (even base profiles), and in the FP64 form the blocks' BF16
-// remainders, which restore the full inverses' values (the blocks alone agree with them to
-// ~1e-11 after a step, enough for the mixed form only).
+// rest), their parity blocks (even base profiles), and in the FP64 form the BF16 remainders
+// of the v blocks, which restore the full inverses' results (the blocks alone agree with them
+// to ~1e-11 after a step, enough for the mixed form only; the eta blocks need none).
 SlotSizes slot_sizes(const Dims& d, int precision) {
     SlotSizes s;
     const long P = d.P, nz = d.nz, zld = (d.hnx + 7) & ~7;
@@ -205,7 +205,7 @@ SlotSizes slot_sizes(const Dims& d, int precision) {
     s.Wl = 5 * P * nz;
     s.M = 2L * d.nslot * nz * nz;
     s.MH = 2L * d.nslot * nz * (d.N / 2 + 1);  // nz rows x (N/2 + 1) pair columns
-    s.ME = precision == 0 ? s.MH : 0;
+    s.ME = precision == 0 ? (long)d.nslot * nz * (d.N / 2 + 1) : 0;  // the v blocks only
     s.Tf = 5 * 2 * RT * S * 64;
     s.Q = nz * nz;
     s.U = nz;

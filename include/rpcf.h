@@ -48,10 +48,10 @@ typedef struct rpcf_ctx rpcf_ctx;
  *                 pressure-gradient correction keeps the bulk of u0 at its value at set_state.
  *   precision     0: FP64 everywhere (reference REAL_DOUBLE, include/data.h:11); the HIP
  *                 library solves an even base profile (u_profile[1] == 0) with the
- *                 reflection-parity blocks of the inverses (half their bytes) plus the
- *                 blocks' remainders in BF16 (an eighth), which restore the full inverses'
- *                 values: the state agrees with the full-inverse solve to ~1e-13 after one
- *                 step (environment RPCF_FULL_INVERSE=1 keeps the full inverses).
+ *                 reflection-parity blocks of the inverses (half their bytes) plus the v
+ *                 blocks' remainders in BF16 (a sixteenth), which restore the full
+ *                 inverses' results: the state agrees with the full-inverse solve to ~1e-13
+ *                 after one step (environment RPCF_FULL_INVERSE=1 keeps the full inverses).
  *                 1: mixed (HIP library; the matrix-core form for 2(nz-1) <= 384) -- the
  *                 transform intermediates, the x and span FFTs and the Lamb vector in FP32,
  *                 the wall-normal Chebyshev transforms as split-FP16 matrix-core products

@@ -61,18 +61,20 @@ def test_hip_library_exports_the_static_api():
 def test_slot_bytes():
     """device bytes of one slot: the per-slot arrays, which the inverses dominate -- the full
     pair, their parity blocks (nz x (N/2 + 1) complex FP64 each) and, in the FP64 form, the
-    blocks' BF16 remainders (4 B per entry)"""
+    BF16 remainders (4 B per entry) of the v blocks"""
     dll = rpcf.load(rpcf.HIP_LIB)
     prm = params("C1")
     b = rpcf.slot_bytes(dll, prm)
     s = rpcf.sizes(dll, prm)
     nslot = s["hnx"] * (s["ny"] // 2 + 1)
+    np_ = (s["nz"] - 1) // 2 + 1
     minv = 2 * nslot * s["nz"] ** 2 * 16
-    blocks = 2 * nslot * s["nz"] * ((s["nz"] - 1) // 2 + 1) * 16
-    assert minv + blocks * 5 // 4 < b < 1.3 * (minv + blocks * 5 // 4)
+    blocks = 2 * nslot * s["nz"] * np_ * 16
+    rem = nslot * s["nz"] * np_ * 4
+    assert minv + blocks + rem < b < 1.35 * (minv + blocks + rem)  # the fields add ~1 GB
     prm.precision = 1  # no remainders
     b1 = rpcf.slot_bytes(dll, prm)
-    assert b - b1 >= blocks // 4
+    assert b - b1 == rem
     with pytest.raises(rpcf.RpcfError):
         rpcf.slot_bytes(dll, params("C0", mz=130))
 

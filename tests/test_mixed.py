@@ -125,7 +125,8 @@ def test_group_precision_must_match():
 @pytest.mark.parametrize("name", ["C1", "DEEP"])
 def test_fp64_solve_remainders_restore_full_inverse(name, monkeypatch):
     """precision 0 with an even base profile solves with the parity blocks and their BF16
-    remainders (5/8 of the full inverses' bytes), which restore the full inverses' values:
+    remainders (the v blocks only; 9/16 of the full inverses' bytes in all), which restore
+    the full inverses' results:
     within 2e-13 of the full-inverse state (RPCF_FULL_INVERSE=1) after one step and 1e-11
     after 20, and at least 50 times closer to it than the blocks alone (precision 2) after
     one step: the BF16 rounding (2^-9 relative) of the remainders leaves ~2^-9 of the gap
