@@ -219,6 +219,41 @@ def test_longest_compiled_chebyshev_step_parity():
     g.close(), o.close()
 
 
+# an even base profile at odd N = 45 (nz = 46): the parity pairs of the solves have no centre
+# row, and 2N = 90, 4N = 180 run the runtime FFT plans and the generic transform kernels
+ODDN = dict(mx=24, my=24, mz=180, n_pi_x=2.0, n_pi_y=1.0, Re=1500.0, Ro=0.1, dt=0.002,
+            u_profile=(1.0, 0.0, -1.0), flow_control=1)
+
+
+def test_channel_odd_n_step_parity(monkeypatch):
+    """the parity-block solves with the v remainders at odd N: one step to 1e-12 and ten to
+    1e-10 against the oracle, ten within 1e-12 of the full-inverse solve (the mixed form runs
+    compiled Chebyshev lengths only, all of even N)"""
+    prm = rpcf.make_params(**ODDN)
+    g, o = pair(prm)
+    v0, e0 = rpcf.seed_state(o.dll, prm, 5, 0.05, 8, 8, 0.0)
+    for sim in (g, o):
+        sim.set_state(v0, e0)
+        sim.bootstrap()
+    a, b = g.get_state(), o.get_state()
+    print("odd N 1 step", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-12 and rel(a[1], b[1]) < 1e-12
+    g.step(9), o.step(9)
+    a, b = g.get_state(), o.get_state()
+    print("odd N 10 steps", rel(a[0], b[0]), rel(a[1], b[1]))
+    assert rel(a[0], b[0]) < 1e-10 and rel(a[1], b[1]) < 1e-10
+    g.close(), o.close()
+    monkeypatch.setenv("RPCF_FULL_INVERSE", "1")
+    with rpcf.Simulation(rpcf.HIP_LIB, prm) as f:
+        f.set_state(v0, e0)
+        f.bootstrap()
+        f.step(9)
+        c = f.get_state()
+    monkeypatch.delenv("RPCF_FULL_INVERSE")
+    print("odd N remainders vs full inverse after 10 steps", rel(a[0], c[0]), rel(a[1], c[1]))
+    assert rel(a[0], c[0]) < 1e-12 and rel(a[1], c[1]) < 1e-12
+
+
 # one grid per compiled Chebyshev length of the FP64 matrix-core p2s (2N = 24 .. 384)
 P2S_GRIDS = {
     "2N24": (dict(mx=24, my=24, mz=48, n_pi_x=2.0, n_pi_y=1.0, Re=800.0, Ro=0.1, dt=0.002,
