@@ -54,8 +54,8 @@ def kernel_bytes(s, prm):
     otherwise the full inverses (nz x nz).  Mixed form (precision 1; channel profile, matrix-core
     Chebyshev transforms): the velocity and vorticity fields, the padded intermediates and
     the Lamb vector in complex FP32 (8 B), the state, nonlinear terms and operator outputs in
-    FP64, and the solves with the parity blocks alone.  Precision 2: FP64 with the parity
-    blocks alone.  (The mean-flow slot reads its two full inverses in every form: 0.2 MB
+    FP64, and the solves with the parity blocks alone, the eta blocks as FP32 high and BF16
+    low parts.  Precision 2: FP64 with the parity blocks alone.  (The mean-flow slot reads its two full inverses in every form: 0.2 MB
     more at 128x129x128, not counted.)"""
     P, nz, hnx, ny, pz = s["P"], s["nz"], s["hnx"], s["ny"], s["pz"]
     my, nxh = prm.my, s["nx"] // 2
@@ -69,8 +69,9 @@ def kernel_bytes(s, prm):
     Zb = pz * my * hnx * 16 * h        # one Lamb component after the x transform
     np_ = (nz - 1) // 2 + 1            # parity pair columns
     M2 = 2 * nz * nz * 16              # a slot's two inverses
-    if parity:  # their parity blocks: nz rows x (N/2 + 1) pair columns, FP64, and the v remainders
-        M2 = 2 * nz * np_ * 16 + (nz * np_ * 4 if prm.precision == 0 else 0)
+    if parity:  # their parity blocks: nz rows x (N/2 + 1) pair columns, FP64, and the v remainders;
+        # mixed form: the eta blocks as FP32 high and BF16 low parts (12 B)
+        M2 = {0: 2 * nz * np_ * 16 + nz * np_ * 4, 1: nz * np_ * (16 + 12)}.get(prm.precision, 2 * nz * np_ * 16)
     nslot = hnx * (ny // 2 + 1)        # matrix slots: pencils +-ky share one
     out = {
         "k_cheb_s2p": 6 * F + 6 * Sp,

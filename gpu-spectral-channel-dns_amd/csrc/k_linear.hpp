@@ -267,7 +267,9 @@ struct LinState {
     double2* se;          // eta-hat (w0 in pencil 0)
     const double2* minv;  // [2][nslot][nz*nz] column-major inverses
     const double2* minvh; // [2][nslot][np][nz] their parity blocks (k_parity_blocks)
-    const unsigned* minve; // [nslot][np][nz] the v blocks' remainders, BF16 (re, im) pairs
+    const unsigned* minve; // [nslot][np][nz] FP64 form: the v blocks' remainders; mixed form: the
+                           // low parts of the eta blocks (BF16 (re, im) pairs)
+    const float2* minvf;   // [nslot][np][nz] mixed form: the high parts of the eta blocks (FP32)
     double qw_target_re, qw_target_im;  // flow-rate target (flow_control)
     int flow_control;
     int parity;           // 1: solve with the parity blocks, 2: with the blocks and remainders
@@ -709,6 +711,66 @@ __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const un
     }
 }
 
+// gemv_sel (no remainders) with the matrix as FP32 high parts Mf and BF16 low parts Ml
+// (same layout); each entry enters as the exact double hi + lo.
+__device__ __forceinline__ void gemv_sel_hilo(const float2* __restrict__ Mf, const unsigned* __restrict__ Ml,
+                                              const double2* xa0, const double2* xa1, const double2* xb0,
+                                              const double2* xb1, int rows, int cols, int split, double2* part,
+                                              int tid, int nthr, double2* y0, double2* y1, int j0) {
+    constexpr int B = kGemvBatch;
+    const int G = nthr / rows;
+    const int g = tid / rows, r = tid - g * rows;
+    const double2* x0 = r < split ? xa0 : xb0;
+    const double2* x1 = r < split ? xa1 : xb1;
+    auto entry = [](float2 h, unsigned l) {
+        const double2 lo = bf16x2_to_double2(l);
+        return make_double2((double)h.x + lo.x, (double)h.y + lo.y);
+    };
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (g < G) {
+        int j = j0 + g;
+        for (; j + G * (B - 1) < cols; j += G * B) {
+            float2 h[B];
+            unsigned l[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                h[u] = Mf[(long)(j + G * u) * rows + r];
+                l[u] = Ml[(long)(j + G * u) * rows + r];
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) gemv_acc(entry(h[u], l[u]), x0[j + G * u], x1[j + G * u], a);
+        }
+        if (j < cols) {
+            float2 h[B];
+            unsigned l[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                h[u] = Mf[(long)min(j + G * u, cols - 1) * rows + r];
+                l[u] = Ml[(long)min(j + G * u, cols - 1) * rows + r];
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int jj = min(j + G * u, cols - 1);
+                const double k = j + G * u < cols ? 1.0 : 0.0;
+                const double2 u0 = x0[jj], u1 = x1[jj];
+                gemv_acc(entry(h[u], l[u]), make_double2(u0.x * k, u0.y * k), make_double2(u1.x * k, u1.y * k), a);
+            }
+        }
+        part[tid] = make_double2(a[0], a[1]);
+        part[kSolveThreads + tid] = make_double2(a[2], a[3]);
+    }
+    __syncthreads();
+    if (tid < rows) {
+        double2 s0 = part[tid], s1 = part[kSolveThreads + tid];
+        for (int q = 1; q < G; q++) {
+            s0 = zadd(s0, part[q * rows + tid]);
+            s1 = zadd(s1, part[kSolveThreads + q * rows + tid]);
+        }
+        *y0 = s0;
+        *y1 = s1;
+    }
+}
+
 // -------------------------------------------------------------------------------------
 // Parity blocks of the inverses of one slot (block (matrix slot, equation)).  With an even
 // base profile the collocation system A of either equation commutes with the wall reflection:
@@ -733,7 +795,13 @@ __device__ __forceinline__ void gemv_sel(const double2* __restrict__ M, const un
 // recovery takes, more), but an eta solve and its first two y-derivatives by <= 4e-15 (the
 // eta system is well conditioned), so only the v blocks keep remainders (same layout as H):
 // the solves read (2 x 16 + 4)/64 of the full inverses' bytes and keep their results.
-// minve (the v remainders) may be null.
+//
+// The mixed form (bar 1e-6) solves with the blocks alone, the eta blocks split into an FP32
+// high part and a BF16 low part (hi + lo = H to ~2^-33 relative, 12 of 16 bytes): the eta
+// system is well conditioned, so its solve stays within ~1e-10 of the FP64 blocks'; the v
+// system (condition ~5e7 at 128x129x128) keeps FP64 blocks.
+// minve: the v remainders (FP64 form, minvf null) or the eta low parts (mixed form, with
+// minvf the eta high parts); both may be null.
 // -------------------------------------------------------------------------------------
 __device__ __forceinline__ unsigned bf16_bits(double x) {
     unsigned u = __float_as_uint((float)x);
@@ -741,11 +809,14 @@ __device__ __forceinline__ unsigned bf16_bits(double x) {
     return u >> 16;
 }
 __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __restrict__ minvh,
-                                unsigned* __restrict__ minve, const int4* __restrict__ pairs, int np, Dims d) {
+                                unsigned* __restrict__ minve, float2* __restrict__ minvf,
+                                const int4* __restrict__ pairs, int np, Dims d) {
     const int slot = blockIdx.x, eq = blockIdx.y, n = d.nz, ne = d.N / 2 + 1;
     const double2* M = minv + ((long)eq * d.nslot + slot) * n * n;  // M[j][i] at i*n + j
     double2* Hm = minvh + ((long)eq * d.nslot + slot) * (long)n * np;
-    unsigned* Em = minve && eq == 0 ? minve + (long)slot * n * np : nullptr;
+    unsigned* Em = minve && !minvf && eq == 0 ? minve + (long)slot * n * np : nullptr;  // v remainders
+    unsigned* Lm = minve && minvf && eq == 1 ? minve + (long)slot * n * np : nullptr;   // eta low parts
+    float2* Fm = minvf && eq == 1 ? minvf + (long)slot * n * np : nullptr;               // eta high parts
     const int4* pr = pairs + eq * np;
     for (long e = threadIdx.x; e < (long)n * np; e += blockDim.x) {
         const int p = (int)(e / n), r = (int)(e - (long)p * n);
@@ -765,6 +836,11 @@ __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __res
         }
         Hm[e] = v;
         if (Em) Em[e] = bf16_bits(w.x) | (bf16_bits(w.y) << 16);
+        if (Fm) {
+            const float2 hi = make_float2((float)v.x, (float)v.y);
+            Fm[e] = hi;
+            Lm[e] = bf16_bits(v.x - (double)hi.x) | (bf16_bits(v.y - (double)hi.y) << 16);
+        }
     }
 }
 
@@ -817,13 +893,13 @@ __device__ __forceinline__ void gemv_t0_reflect(const double* __restrict__ T0t, 
 // for a member with an even base profile, with its parity blocks H (k_parity_blocks): the
 // combined sides b_i + s b_i' and b_i - s b_i' per row pair, then one nz x np GEMV whose even
 // rows take the first and odd rows the second (parity 2 and Em given: and the remainders E
-// the other).  b vanishes on the rows below j0 (an even
+// the other; Hf given: the blocks as FP32 high parts Hf and BF16 low parts Em).  b vanishes on the rows below j0 (an even
 // count: whole row pairs), whose columns are skipped.  Thread r < nz returns x at
 // coefficient *jrow.  Every thread of the block calls it (it synchronises).
-__device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const unsigned* Em, const int4* pairs,
-                                           int np, int parity, double2 (*rhs)[256], double2 (*bc)[2][128],
-                                           double2* part, int n, int N, int tid, int nthr, double2* y0, double2* y1,
-                                           int* jrow, int j0) {
+__device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const unsigned* Em, const float2* Hf,
+                                           const int4* pairs, int np, int parity, double2 (*rhs)[256],
+                                           double2 (*bc)[2][128], double2* part, int n, int N, int tid, int nthr,
+                                           double2* y0, double2* y1, int* jrow, int j0) {
     if (!parity) {
         gemv_grouped(M, rhs[0], rhs[1], n, part, tid, nthr, y0, y1, j0);
         *jrow = tid;
@@ -845,7 +921,9 @@ __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, 
     }
     __syncthreads();
     const int ne = N / 2 + 1;
-    if (parity == 2 && Em)
+    if (Hf)
+        gemv_sel_hilo(Hf, Em, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
+    else if (parity == 2 && Em)
         gemv_sel<true>(Hm, Em, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
     else
         gemv_sel<false>(Hm, Em, bc[0][0], bc[1][0], bc[0][1], bc[1][1], n, np, ne, part, tid, nthr, y0, y1, j0 / 2);
@@ -869,6 +947,7 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
                                                    const double2* __restrict__ MINV, long sM,
                                                    const double2* __restrict__ MINVH, long sMH,
                                                    const unsigned* __restrict__ MINVE, long sME,
+                                                   const float2* __restrict__ MINVF,
                                                    const int4* __restrict__ pairs, int np,
                                                    const double2* __restrict__ W, long sWl, Dims d, Members mb,
                                                    const double* __restrict__ T0t, const double* __restrict__ U,
@@ -894,6 +973,7 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
         st.minv = MINV + ms * sM;
         st.minvh = MINVH + ms * sMH;
         st.minve = MINVE ? MINVE + ms * sME : nullptr;
+        st.minvf = MINVF ? MINVF + ms * sME : nullptr;
         st.parity = mp.parity;
         st.qw_target_re = mp.q_re;
         st.qw_target_im = mp.q_im;
@@ -992,9 +1072,9 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
     {
         double2 y0, y1;
         int j;
-        const unsigned* Ev = st.minve ? st.minve + (long)slot * nh : nullptr;
-        solve_slot(Mv, st.minvh + (long)slot * nh, Ev, pairs, np, st.parity, rhs, bc, &part[0][0], n, d.N, r,
-                   blockDim.x, &y0, &y1, &j, 4);
+        const unsigned* Ev = st.minve && !st.minvf ? st.minve + (long)slot * nh : nullptr;
+        solve_slot(Mv, st.minvh + (long)slot * nh, Ev, nullptr, pairs, np, st.parity, rhs, bc, &part[0][0], n, d.N,
+                   r, blockDim.x, &y0, &y1, &j, 4);
         if (r < n) vnew[0][j] = y0, vnew[1][j] = y1;
     }
     __syncthreads();
@@ -1023,8 +1103,11 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
     __syncthreads();
     double2 e0, e1;
     int je;
-    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, nullptr, pairs + np, np, st.parity, rhs, bc, &part[0][0],
-               n, d.N, r, blockDim.x, &e0, &e1, &je, 2);  // eta: the blocks without remainders
+    // eta: the FP64 blocks without remainders, or (mixed form) their FP32 + BF16 split
+    const bool split = st.minvf && st.parity == 1;
+    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, split ? st.minve + (long)slot * nh : nullptr,
+               split ? st.minvf + (long)slot * nh : nullptr, pairs + np, np, st.parity, rhs, bc, &part[0][0], n, d.N,
+               r, blockDim.x, &e0, &e1, &je, 2);
     if (r < n) {
         st.se[po0 + je] = e0;
         st.sv[po0 + r] = vnew[0][r];

@@ -107,7 +107,9 @@ struct rpcf_ctx {
     double2* WL = nullptr;    // [5][P][nz] operator products of the linear right-hand sides
     double2* MINV = nullptr;  // [2][nslot][nz*nz] inverses, column-major
     double2* MINVH = nullptr; // [2][nslot][np][nz] parity blocks of the inverses (even base profiles)
-    unsigned* MINVE = nullptr; // [nslot][np][nz] the v blocks' remainders, BF16 (re, im) (FP64 form)
+    unsigned* MINVE = nullptr; // [nslot][np][nz] BF16 (re, im): the v blocks' remainders (FP64 form)
+                               // or the eta blocks' low parts (mixed form)
+    float2* MINVF = nullptr;   // [nslot][np][nz] the eta blocks' FP32 high parts (mixed form)
     int4* PAIRS = nullptr;    // [2][np] row pairs (i, i', sign, 0) of the v and eta systems
     int npair = 0;            // np = N/2 + 1
     double* TF = nullptr;     // [5][RT][S][64] MFMA A-operand tiles of T0, T2, T4q, T0, T2q
@@ -181,9 +183,10 @@ int check_sizes(const rpcf_params* prm, Dims* out) {
 
 // per-slot element counts of every per-slot array (= the slot strides)
 struct SlotSizes {
-    long F, S, NL, Y, W, Z, Wl, M, MH, ME, Tf, Q, U, Pz;
+    long F, S, NL, Y, W, Z, Wl, M, MH, ME, MF, Tf, Q, U, Pz;
     size_t bytes() const {
         return sizeof(double2) * (size_t)(F + 2 * S + NL + Y + W + Z + Wl + M + MH) + sizeof(unsigned) * (size_t)ME +
+               sizeof(float2) * (size_t)MF +
                sizeof(double) * (size_t)(Tf + 2 * Q + 3 * U + Pz) + sizeof(MemberParams) + 2 * sizeof(int);
     }
 };
@@ -191,7 +194,8 @@ struct SlotSizes {
 // Inverses: the full ones (the mean flow's slot, odd base profiles, and the source of the
 // rest), their parity blocks (even base profiles), and in the FP64 form the BF16 remainders
 // of the v blocks, which restore the full inverses' results (the blocks alone agree with them
-// to ~1e-11 after a step, enough for the mixed form only; the eta blocks need none).
+// to ~1e-11 after a step, enough for the mixed form only; the eta blocks need none); in the
+// mixed form the eta blocks also as FP32 high and BF16 low parts, which its solves read.
 SlotSizes slot_sizes(const Dims& d, int precision) {
     SlotSizes s;
     const long P = d.P, nz = d.nz, zld = (d.hnx + 7) & ~7;
@@ -205,7 +209,8 @@ SlotSizes slot_sizes(const Dims& d, int precision) {
     s.Wl = 5 * P * nz;
     s.M = 2L * d.nslot * nz * nz;
     s.MH = 2L * d.nslot * nz * (d.N / 2 + 1);  // nz rows x (N/2 + 1) pair columns
-    s.ME = precision == 0 ? (long)d.nslot * nz * (d.N / 2 + 1) : 0;  // the v blocks only
+    s.ME = precision <= 1 ? (long)d.nslot * nz * (d.N / 2 + 1) : 0;  // v remainders / eta low parts
+    s.MF = precision == 1 ? s.ME : 0;                                  // eta high parts
     s.Tf = 5 * 2 * RT * S * 64;
     s.Q = nz * nz;
     s.U = nz;
@@ -728,7 +733,8 @@ int enq_lin_solve(rpcf_ctx* c, const int* ids, int n, bool cap = false) {
             // about four column groups of nz threads (all of kSolveThreads at nz = 129)
             const int thr = std::min(kSolveThreads, 64 * ((4 * d.nz + 63) / 64));
             k_lin_solve<<<dim3(d.nslot, n), thr, 0, c->stream>>>(c->SV, c->SE, c->sS, c->NL, c->sNL, c->MINV, c->sM,
-                                                                 c->MINVH, c->sMH, c->MINVE, c->sME, c->PAIRS, c->npair,
+                                                                 c->MINVH, c->sMH, c->MINVE, c->sME, c->MINVF, c->PAIRS,
+                                                                 c->npair,
                                                                  c->WL, c->sWl, d, members(c, ids), c->T0t, c->U, c->DU,
                                                                  c->DDU, c->sU, c->qw);
         },
@@ -875,7 +881,7 @@ void free_ctx(rpcf_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     void* ptrs[] = {c->F,   c->SV, c->SE, c->NL,  c->Y,     c->W,  c->Z2, c->WL, c->MINV, c->TF,  c->T4Q,
                     c->T2Q, c->U,  c->DU, c->DDU, c->MEANU, c->MP, c->PH, c->T0, c->T2,   c->T4,  c->T0t,
-                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->MINVE, c->PAIRS, c->MT, c->DT};
+                    c->T2t, c->qw, c->tw, c->ids_dev, c->ids_zero, c->twf, c->MINVH, c->MINVE, c->MINVF, c->PAIRS, c->MT, c->DT};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -956,7 +962,7 @@ int init_ctx(rpcf_ctx* c) {
         (rc = dalloc(c, &c->NL, C * ss.NL)) || (rc = dalloc(c, &c->Y, C * ss.Y)) || (rc = dalloc(c, &c->W, C * ss.W)) ||
         (rc = dalloc(c, &c->Z2, C * ss.Z)) || (rc = dalloc(c, &c->WL, C * ss.Wl)) ||
         (rc = dalloc(c, &c->MINV, C * ss.M)) || (rc = dalloc(c, &c->MINVH, C * ss.MH)) ||
-        (ss.ME && (rc = dalloc(c, &c->MINVE, C * ss.ME))) ||
+        (ss.ME && (rc = dalloc(c, &c->MINVE, C * ss.ME))) || (ss.MF && (rc = dalloc(c, &c->MINVF, C * ss.MF))) ||
         (rc = dalloc(c, &c->TF, C * ss.Tf)) ||
         (rc = dalloc(c, &c->T4Q, C * ss.Q)) || (rc = dalloc(c, &c->T2Q, C * ss.Q)) ||
         (rc = dalloc(c, &c->U, C * ss.U)) || (rc = dalloc(c, &c->DU, C * ss.U)) || (rc = dalloc(c, &c->DDU, C * ss.U)) ||
@@ -1172,6 +1178,7 @@ int build_parity_blocks(rpcf_ctx* c, int slot) {
     const Dims d = c->d;
     k_parity_blocks<<<dim3(d.nslot, 2), 256, 0, c->stream>>>(c->MINV + slot * c->sM, c->MINVH + slot * c->sMH,
                                                              c->MINVE ? c->MINVE + slot * c->sME : nullptr,
+                                                             c->MINVF ? c->MINVF + slot * c->sME : nullptr,
                                                              c->PAIRS, c->npair, d);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? RPCF_OK : fail(c, RPCF_EDEVICE, "k_parity_blocks: %s", hipGetErrorString(e));
@@ -1929,6 +1936,7 @@ int rpcf_member_adopt(rpcf_ctx* c, int slot, rpcf_ctx* src, int src_slot) {
     cp(c->MINV, src->MINV, c->sM);
     cp(c->MINVH, src->MINVH, c->sMH);
     cp(c->MINVE, src->MINVE, c->sME);
+    cp(c->MINVF, src->MINVF, c->MINVF ? c->sME : 0);
     cp(c->TF, src->TF, c->sTf);
     cp(c->T4Q, src->T4Q, c->sQ);
     cp(c->T2Q, src->T2Q, c->sQ);

@@ -72,9 +72,12 @@ def test_slot_bytes():
     blocks = 2 * nslot * s["nz"] * np_ * 16
     rem = nslot * s["nz"] * np_ * 4
     assert minv + blocks + rem < b < 1.35 * (minv + blocks + rem)  # the fields add ~1 GB
-    prm.precision = 1  # no remainders
+    prm.precision = 2  # the parity blocks alone
+    b2 = rpcf.slot_bytes(dll, prm)
+    assert b - b2 == rem
+    prm.precision = 1  # the eta blocks also as FP32 high and BF16 low parts (12 B per entry)
     b1 = rpcf.slot_bytes(dll, prm)
-    assert b - b1 == rem
+    assert b1 - b2 == 3 * rem
     with pytest.raises(rpcf.RpcfError):
         rpcf.slot_bytes(dll, params("C0", mz=130))
 
