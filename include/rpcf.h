@@ -58,7 +58,8 @@ typedef struct rpcf_ctx rpcf_ctx;
  *                 with FP32 accumulation; velocity recovery, the nonlinear terms, the
  *                 operators and the solves in FP64, an even base profile's solves with
  *                 the parity blocks alone (they agree with the full inverses to the
- *                 inverse's own round-off asymmetry).
+ *                 inverse's own round-off asymmetry; the eta blocks stored as FP32 high
+ *                 and BF16 low parts, ~2^-33 relative).
  *                 2: FP64 with the parity blocks alone (even base profiles; an odd profile
  *                 runs as 0): the state agrees with precision 0 and the reference to about
  *                 1e-11 after one step.  The oracle computes FP64 with full inverses
