@@ -266,7 +266,7 @@ struct LinState {
     double2* sv;          // v-hat (u0 in pencil 0), updated in place
     double2* se;          // eta-hat (w0 in pencil 0)
     const double2* minv;  // [2][nslot][nz*nz] column-major inverses
-    const double2* minvh; // [2][nslot][np][nz] their parity blocks (k_parity_blocks)
+    const double2* minvh; // [2][nslot][np][nz] their parity blocks (k_parity_blocks; mixed form: v only)
     const unsigned* minve; // [nslot][np][nz] FP64 form: the v blocks' remainders; mixed form: the
                            // low parts of the eta blocks (BF16 (re, im) pairs)
     const float2* minvf;   // [nslot][np][nz] mixed form: the high parts of the eta blocks (FP32)
@@ -813,7 +813,8 @@ __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __res
                                 const int4* __restrict__ pairs, int np, Dims d) {
     const int slot = blockIdx.x, eq = blockIdx.y, n = d.nz, ne = d.N / 2 + 1;
     const double2* M = minv + ((long)eq * d.nslot + slot) * n * n;  // M[j][i] at i*n + j
-    double2* Hm = minvh + ((long)eq * d.nslot + slot) * (long)n * np;
+    // the mixed form keeps no FP64 eta blocks (its solves read the split parts)
+    double2* Hm = minvf && eq == 1 ? nullptr : minvh + ((long)eq * d.nslot + slot) * (long)n * np;
     unsigned* Em = minve && !minvf && eq == 0 ? minve + (long)slot * n * np : nullptr;  // v remainders
     unsigned* Lm = minve && minvf && eq == 1 ? minve + (long)slot * n * np : nullptr;   // eta low parts
     float2* Fm = minvf && eq == 1 ? minvf + (long)slot * n * np : nullptr;               // eta high parts
@@ -834,7 +835,7 @@ __global__ void k_parity_blocks(const double2* __restrict__ minv, double2* __res
             v = make_double2(0.5 * (a.x + sg * b.x), 0.5 * (a.y + sg * b.y));
             w = make_double2(0.5 * (a.x - sg * b.x), 0.5 * (a.y - sg * b.y));
         }
-        Hm[e] = v;
+        if (Hm) Hm[e] = v;
         if (Em) Em[e] = bf16_bits(w.x) | (bf16_bits(w.y) << 16);
         if (Fm) {
             const float2 hi = make_float2((float)v.x, (float)v.y);
@@ -1105,7 +1106,7 @@ __global__ __launch_bounds__(kSolveThreads, 6) void k_lin_solve(double2* __restr
     int je;
     // eta: the FP64 blocks without remainders, or (mixed form) their FP32 + BF16 split
     const bool split = st.minvf && st.parity == 1;
-    solve_slot(Mo, st.minvh + ((long)d.nslot + slot) * nh, split ? st.minve + (long)slot * nh : nullptr,
+    solve_slot(Mo, split ? nullptr : st.minvh + ((long)d.nslot + slot) * nh, split ? st.minve + (long)slot * nh : nullptr,
                split ? st.minvf + (long)slot * nh : nullptr, pairs + np, np, st.parity, rhs, bc, &part[0][0], n, d.N,
                r, blockDim.x, &e0, &e1, &je, 2);
     if (r < n) {

@@ -106,7 +106,8 @@ struct rpcf_ctx {
     double2* Z2 = nullptr;    // [3][pz][my][zld], zld = hnx rounded up to 8 (128-byte rows)
     double2* WL = nullptr;    // [5][P][nz] operator products of the linear right-hand sides
     double2* MINV = nullptr;  // [2][nslot][nz*nz] inverses, column-major
-    double2* MINVH = nullptr; // [2][nslot][np][nz] parity blocks of the inverses (even base profiles)
+    double2* MINVH = nullptr; // [2][nslot][np][nz] parity blocks of the inverses (even base profiles;
+                              // the mixed form keeps the v blocks only, [1][nslot][np][nz])
     unsigned* MINVE = nullptr; // [nslot][np][nz] BF16 (re, im): the v blocks' remainders (FP64 form)
                                // or the eta blocks' low parts (mixed form)
     float2* MINVF = nullptr;   // [nslot][np][nz] the eta blocks' FP32 high parts (mixed form)
@@ -208,7 +209,7 @@ SlotSizes slot_sizes(const Dims& d, int precision) {
     s.Z = 3L * d.pz * d.my * zld;
     s.Wl = 5 * P * nz;
     s.M = 2L * d.nslot * nz * nz;
-    s.MH = 2L * d.nslot * nz * (d.N / 2 + 1);  // nz rows x (N/2 + 1) pair columns
+    s.MH = (precision == 1 ? 1L : 2L) * d.nslot * nz * (d.N / 2 + 1);  // nz rows x (N/2 + 1) pair columns; mixed: v
     s.ME = precision <= 1 ? (long)d.nslot * nz * (d.N / 2 + 1) : 0;  // v remainders / eta low parts
     s.MF = precision == 1 ? s.ME : 0;                                  // eta high parts
     s.Tf = 5 * 2 * RT * S * 64;

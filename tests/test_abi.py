@@ -75,9 +75,9 @@ def test_slot_bytes():
     prm.precision = 2  # the parity blocks alone
     b2 = rpcf.slot_bytes(dll, prm)
     assert b - b2 == rem
-    prm.precision = 1  # the eta blocks also as FP32 high and BF16 low parts (12 B per entry)
+    prm.precision = 1  # the eta blocks as FP32 high and BF16 low parts (12 B per entry) only
     b1 = rpcf.slot_bytes(dll, prm)
-    assert b1 - b2 == 3 * rem
+    assert b1 - b2 == 3 * rem - blocks // 2
     with pytest.raises(rpcf.RpcfError):
         rpcf.slot_bytes(dll, params("C0", mz=130))
 
