@@ -894,9 +894,9 @@ __device__ __forceinline__ void gemv_t0_reflect(const double* __restrict__ T0t, 
 // for a member with an even base profile, with its parity blocks H (k_parity_blocks): the
 // combined sides b_i + s b_i' and b_i - s b_i' per row pair, then one nz x np GEMV whose even
 // rows take the first and odd rows the second (parity 2 and Em given: and the remainders E
-// the other; Hf given: the blocks as FP32 high parts Hf and BF16 low parts Em).  b vanishes on the rows below j0 (an even
-// count: whole row pairs), whose columns are skipped.  Thread r < nz returns x at
-// coefficient *jrow.  Every thread of the block calls it (it synchronises).
+// the other; Hf given: the blocks as FP32 high parts Hf and BF16 low parts Em).  b vanishes
+// on the rows below j0 (an even count: whole row pairs), whose columns are skipped.  Thread
+// r < nz returns x at coefficient *jrow.  Every thread of the block calls it (it synchronises).
 __device__ __forceinline__ void solve_slot(const double2* M, const double2* Hm, const unsigned* Em, const float2* Hf,
                                            const int4* pairs, int np, int parity, double2 (*rhs)[256],
                                            double2 (*bc)[2][128], double2* part, int n, int N, int tid, int nthr,
