@@ -575,7 +575,7 @@ __global__ __launch_bounds__(64 * NW) void k_lin_tv_stream(const double2* __rest
 // synchronises).
 constexpr int kSolveThreads = 576;
 // matrix loads per thread in flight in the solve's GEMVs: with 4 the solve needs 80 registers
-// and no spills, six waves per SIMD; at 128x129x128 the parity-block solve takes 282 us
+// and no spills, six waves per SIMD; at 128x129x128 the parity-block solve took 282 us
 // against 299 with 8 (its fixed phases weigh more), the full-inverse solve the same 474 us.
 // The summation order is the same for any batch.
 constexpr int kGemvBatch = 4;
