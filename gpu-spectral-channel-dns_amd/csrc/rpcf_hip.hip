@@ -103,7 +103,7 @@ struct rpcf_ctx {
     double2* NL = nullptr;    // [4][P][nz] N_v (2 buffers), N_eta (2 buffers); current = phase
     double2* Y = nullptr;     // [6][hnx][pz][ny] (also Z1 [3][hnx][pz][ny])
     double2* W = nullptr;     // [6][pz][my][nxh]
-    double2* Z2 = nullptr;    // [3][pz][my][zld], zld = hnx rounded up to 8 (128-byte rows)
+    double2* Z2 = nullptr;    // [3][pz][my][zld], zld = hnx rounded up to 8 (128-byte rows; FP32: 16)
     double2* WL = nullptr;    // [5][P][nz] operator products of the linear right-hand sides
     double2* MINV = nullptr;  // [2][nslot][nz*nz] inverses, column-major
     double2* MINVH = nullptr; // [2][nslot][np][nz] parity blocks of the inverses (even base profiles;
@@ -476,8 +476,14 @@ template <typename R>
 int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
     const Dims d = c->d;
     int rc;
+    // rows of Z2 padded to whole 128-byte lines: 8 FP64 or 16 FP32 entries (FP32 rows padded
+    // to 8 entries straddle lines, and the span transform fetched 1.75x its bytes); an FP32
+    // member keeps its whole FP64-sized slot (twice the stride in FP32 entries), so that the
+    // wider padding stays inside it
+    const int zld = sizeof(R) == 4 ? (d.hnx + 15) & ~15 : c->zld;
+    const long sZ = sizeof(R) == 4 ? 2 * c->sZ : c->sZ;
     const long fS = (long)d.P * d.nz, yS = (long)d.hnx * d.pz * d.ny, wS = (long)d.pz * d.my * d.nxh,
-               zS = (long)d.pz * d.my * c->zld;
+               zS = (long)d.pz * d.my * zld;
     const double2* tw4 = (const double2*)c->pl_zp.tw;
     const Members mb = members(c, ids);
     using C2 = cplx<R>;
@@ -562,12 +568,12 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
             c, KID_LAMB,
             [&] {
                 switch (d.mx) {
-#define RPCF_CASE(L) case L: k_lamb_phys<L, R><<<g, kFftThreads, lds, c->stream>>>(W, wS, c->sW, Z2, zS, c->sZ, c->zld, c->MEANU, c->sPz, c->fac, d, plx, RP, ld, mb); break;
+#define RPCF_CASE(L) case L: k_lamb_phys<L, R><<<g, kFftThreads, lds, c->stream>>>(W, wS, c->sW, Z2, zS, sZ, zld, c->MEANU, c->sPz, c->fac, d, plx, RP, ld, mb); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
                         if constexpr (f64)
-                            k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(W, wS, c->sW, Z2, zS, c->sZ, c->zld,
+                            k_lamb_phys<<<g, kFftThreads, lds, c->stream>>>(W, wS, c->sW, Z2, zS, sZ, zld,
                                                                             c->MEANU, c->sPz, c->fac, d, plx, RP, ld,
                                                                             mb);
                 }
@@ -583,12 +589,12 @@ int enq_transforms(rpcf_ctx* c, const int* ids, int n, bool cap) {
             c, KID_SPAN_FWD,
             [&] {
                 switch (d.my) {
-#define RPCF_CASE(L) case L: k_span_fwd<L, R><<<g, kFftThreads, lds, c->stream>>>(Z2, zS, c->sZ, c->zld, Y, yS, c->sY, d, d.pz, ply, TK, ld, ids, c->PH); break;
+#define RPCF_CASE(L) case L: k_span_fwd<L, R><<<g, kFftThreads, lds, c->stream>>>(Z2, zS, sZ, zld, Y, yS, c->sY, d, d.pz, ply, TK, ld, ids, c->PH); break;
                     RPCF_FFT_LENGTHS(RPCF_CASE)
 #undef RPCF_CASE
                     default:
                         if constexpr (f64)
-                            k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(Z2, zS, c->sZ, c->zld, Y, yS, c->sY, d,
+                            k_span_fwd<<<g, kFftThreads, lds, c->stream>>>(Z2, zS, sZ, zld, Y, yS, c->sY, d,
                                                                            d.pz, ply, TK, ld, ids, c->PH);
                 }
             },
